@@ -1,0 +1,182 @@
+"""SSG time-step benchmark on one MI355X.
+
+Workload: the precision configuration's grid, Nx = Ny = 1024 (Nz = 32 carried
+by the domain), surface buoyancy b0 from the seeded random-phase spectrum
+k²·exp(-(k/6)²) at rms 1, frozen φₛ = ε = 0 inversion of b0, RK3 steps of
+dt = 2.5e-4 with the exponential filter (α = 36, p = 8, c = 0.65) every 10
+steps.  A step is one `timestep!` (three tendency evaluations + stage
+updates, plus the filter on every 10th step).
+
+`value` is FP32-mixed RK3 steps/s with all inputs resident in HBM; the plain
+FP64 drop-in is measured in the same run and reported beside it with the
+rel-L2 difference of b between the two.  The roofline object covers the
+dominant kernel class, timed with HIP events on the library's stream in a
+second, instrumented pass over the same number of steps.  The CPU baseline
+is the FP64 oracle (a C restatement of the reference step) on the host
+cores, on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "ssg.jl_amd"))
+
+METRIC = "SSG RK3 steps/s @1024²×32: FP32-mixed vs FP64 at b rel-err≤1e-5; %HBM peak"
+HBM_PEAK_GBS = 8000.0
+KIND_NAMES = {0: "k_rows_fwd", 1: "k_cols", 2: "k_rows_inv"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(ssg, prec, b0, psi, nx, ny, nz, params, warmup, steps, profile):
+    d = ssg.make_domain(nx, ny, nz, precision=prec)
+    f = ssg.allocate_fields(d)
+    f.phis = psi
+    f.bs = b0
+    st = ssg.TimeState(0.0, f)
+    if warmup:
+        ssg.timestep(f, d, params, st, warmup)
+    d.sync()
+    t0 = time.perf_counter()
+    ssg.timestep(f, d, params, st, steps)
+    d.sync()
+    wall = time.perf_counter() - t0
+    gpu_ms, launches = d.last_timing()
+    b_end = f.bs
+    prof = None
+    if profile:
+        d.profile(True)
+        ssg.timestep(f, d, params, st, steps)
+        prof = {k: d.profile_read(k) for k in KIND_NAMES}
+        d.profile(False)
+    d.close()
+    return wall, gpu_ms, launches, b_end, prof
+
+
+def load_traffic(workload):
+    """Measured HBM bytes per launch of the dominant kernel (rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE, scaled by a copy-kernel calibration), if a
+    summary for this workload is committed under profiles/."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as fh:
+            data = json.load(fh)
+        return data.get(workload)
+    except Exception:
+        return None
+
+
+def cpu_baseline(b0, psi, nx, ny, nz, dt, budget_s):
+    from oracle import Oracle, num_threads
+    o = Oracle(nx, ny, nz)
+    b = b0.copy()
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.timestep(b, psi, 1, dt, 10, 36.0, 8, 0.65, 0)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 1000:
+            break
+    return {"value": n / el, "unit": "steps/s", "cores": num_threads(), "kind": "port",
+            "sample": f"{n} RK3 steps of the same {nx}x{ny} workload (FP64 C oracle, OpenMP), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--nz", type=int, default=32)
+    ap.add_argument("--dt", type=float, default=2.5e-4)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "fp32"], default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is a design, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    import ssgamd as ssg
+    from ssgamd.ics import spectral_b0, streamfunction_from_b
+
+    nx = ny = args.nx
+    nz = args.nz
+    b0 = spectral_b0(nx, ny, seed=0, k0=6.0)
+    psi = streamfunction_from_b(b0)
+    params = ssg.TimeParams(args.dt, scheme=ssg.RK3, filter_freq=10, filter_strength=36.0, filter_order=8,
+                            filter_cutoff=0.65)
+    res = {}
+    for name, prec in (("fp64", ssg.SSG_FP64), ("fp32", ssg.SSG_FP32_MIXED)):
+        if args.only and name != args.only:
+            continue
+        log(f"bench: {name} RK3 {nx}x{ny} warmup={args.warmup} steps={args.steps}")
+        res[name] = run_gpu(ssg, prec, b0, psi, nx, ny, nz, params, args.warmup, args.steps,
+                            profile=(not args.no_profile) and name == "fp32")
+        log(f"bench: {name} {args.steps / res[name][0]:.1f} steps/s (wall), gpu {res[name][1]:.2f} ms")
+
+    main_name = "fp32" if "fp32" in res else "fp64"
+    wall, gpu_ms, launches, b_end, prof = res[main_name]
+    value = args.steps / wall
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 state)" if main_name == "fp32" else "f64",
+        "data": "synthetic (seeded random-phase b0, frozen psi)",
+        "config": {"workload": f"RK3 timestep! {nx}x{ny}x{nz} (precision config grid), Mode R (frozen psi), "
+                               f"dt={args.dt}, exp filter a=36 p=8 c=0.65 every 10 steps",
+                   "nx": nx, "ny": ny, "nz": nz, "scheme": "RK3", "mode": "R", "form": main_name},
+    }
+    if "fp64" in res:
+        out["fp64_steps_per_s"] = args.steps / res["fp64"][0]
+    if "fp64" in res and "fp32" in res:
+        out["fp32_over_fp64"] = out["value"] / out["fp64_steps_per_s"]
+        bf, b6 = res["fp32"][3], res["fp64"][3]
+        out["b_rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(bf - b6) / np.linalg.norm(b6))
+        out["b_rel_l2_steps"] = args.warmup + args.steps
+    if prof:
+        kinds = {KIND_NAMES[k]: {"ms": v[0], "launches": v[1], "bytes": v[2]} for k, v in prof.items()}
+        dom = max(kinds, key=lambda k: kinds[k]["ms"])
+        kd = kinds[dom]
+        avg_s = kd["ms"] / 1e3 / max(kd["launches"], 1)
+        per_launch = kd["bytes"] / max(kd["launches"], 1)
+        achieved = per_launch / avg_s / 1e9
+        traffic = load_traffic(f"{dom}:{nx}:{main_name}")
+        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                           "bytes_per_launch": per_launch, "avg_launch_us": avg_s * 1e6}
+        tot_ms = sum(v["ms"] for v in kinds.values())
+        tot_b = sum(v["bytes"] for v in kinds.values())
+        out["kernels"] = {k: {"avg_us": 1e3 * v["ms"] / max(v["launches"], 1), "launches": v["launches"],
+                              "GBps": v["bytes"] / max(v["ms"], 1e-12) / 1e6} for k, v in kinds.items()}
+        out["step_algorithmic_GBps"] = tot_b / (tot_ms / 1e3) / 1e9 if tot_ms > 0 else None
+    if not args.no_cpu:
+        log("bench: CPU baseline (oracle)")
+        dt = args.dt
+        out["cpu_baseline"] = cpu_baseline(b0, psi, nx, ny, nz, dt, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,154 @@
+/*
+ * ssg.h — C ABI of the MI355X SSG time-step library (libssg_hip.so).
+ *
+ * Drop-in for the time-step hot path of SSG.jl: the reference's
+ * `timestep!(fields, domain, params, state)` and the spectral routines under
+ * it.  A Julia host binds these with `ccall`; the
+ * Python mirror in ssg.jl_amd/ssgamd binds them with ctypes.
+ *
+ * Conventions
+ *   - Real 2D fields cross the ABI as FP64, column-major with x fastest:
+ *     element (i, j) at i + nx*j (src/fields.jl:30-60).
+ *   - Reference-layout spectra are (nx, ny/2+1) interleaved complex FP64,
+ *     x fastest (src/domain.jl:117-141).
+ *   - Every call returns 0 on success and a negative SSG_ERR_* code on
+ *     failure; ssg_last_error() gives the message.  No C++ exception,
+ *     abort or exit crosses the ABI.
+ *   - The context owns all device memory.  Host pointers are borrowed for
+ *     the duration of a call.  A context is used by one host thread at a
+ *     time; calls are synchronous.
+ *   - Grid limits of this build: nx and ny powers of two in [8, 4096].
+ */
+#ifndef SSG_H
+#define SSG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ssg_ctx ssg_ctx;
+
+enum {
+    SSG_OK = 0,
+    SSG_ERR_ARG = -1,
+    SSG_ERR_HIP = -2,
+    SSG_ERR_NONFINITE = -3,
+    SSG_ERR_UNSUPPORTED = -4,
+    SSG_ERR_NOT_CONVERGED = -5
+};
+
+/* arithmetic of the hot path */
+enum { SSG_FP64 = 0, SSG_FP32_MIXED = 1 };
+
+/* TimeScheme, src/timestep.jl:10-14 (same integer values) */
+enum { SSG_AB2_LOWSTORAGE = 0, SSG_RK3 = 1, SSG_RK3_LOWSTORAGE = 2 };
+
+/* field identifiers for ssg_set_field / ssg_get_field */
+enum {
+    SSG_FIELD_B = 0,   /* fields.bₛ  (nx*ny)            src/fields.jl:30 */
+    SSG_FIELD_PHI = 1, /* fields.φₛ  (nx*ny)            src/fields.jl:31 */
+    SSG_FIELD_U = 2,   /* surface u = -∂yφₛ (get only)  src/setICs.jl:161-192 */
+    SSG_FIELD_V = 3    /* surface v =  ∂xφₛ (get only) */
+};
+
+/* Domain, src/domain.jl:33-74 (uniform z grid, src/domain.jl:238-249) */
+typedef struct {
+    int32_t nx, ny, nz;
+    double lx, ly, lz;
+    int32_t precision; /* SSG_FP64 or SSG_FP32_MIXED */
+    int32_t device;    /* HIP device ordinal */
+} ssg_domain_desc;
+
+/* TimeParams, src/timestep.jl:19-40, plus the filter shape that the
+   reference fixes in its legacy path (order 4, cutoff 0.65,
+   src/filter.jl:525-532) */
+typedef struct {
+    double dt;
+    int32_t scheme;
+    int32_t filter_freq;            /* 0 = never */
+    double filter_strength;         /* α */
+    int32_t filter_order;           /* p (reference legacy path: 4) */
+    double filter_cutoff;           /* c (reference legacy path: 0.65) */
+    int32_t ab2_reference_aliasing; /* 1: reproduce the tendency of
+                                       src/timestep.jl:159 exactly */
+    int32_t check_finite;           /* 1: fail with SSG_ERR_NONFINITE if b
+                                       is not finite after the call */
+} ssg_time_params;
+
+/* TimeState scalars, src/timestep.jl:45-86 (the stage buffers and the AB2
+   history live in the context) */
+typedef struct {
+    double t;
+    int64_t step;
+    double dt_actual;
+} ssg_time_state;
+
+/* FP64 diagnostics */
+typedef struct {
+    double sum_b;          /* Σ b */
+    double sum_b2;         /* Σ b² */
+    double parseval_b;     /* parsevalsum2(rfft2(b)), src/transforms.jl:230-274 */
+    double ke_surface;     /* compute_surface_kinetic_energy, src/setICs.jl:225-246 */
+    int64_t nonfinite;     /* count of non-finite b entries */
+} ssg_diag;
+
+/* make_domain + allocate_fields + TimeState (src/domain.jl:217-230,
+   src/fields.jl:72-116, src/timestep.jl:45-86) */
+int ssg_create(ssg_ctx** out, const ssg_domain_desc* desc);
+int ssg_destroy(ssg_ctx* ctx);
+const char* ssg_last_error(const ssg_ctx* ctx);
+int ssg_sync(ssg_ctx* ctx);
+
+/* copy a field in / out; n = nx*ny.  Setting PHI recomputes the surface
+   velocities (src/setICs.jl:161-192).  Setting B resets the AB2 history. */
+int ssg_set_field(ssg_ctx* ctx, int32_t field, const double* host, int64_t n);
+int ssg_get_field(ssg_ctx* ctx, int32_t field, double* host, int64_t n);
+/* same with device pointers (FP64, nx*ny), stream-ordered on the context */
+int ssg_set_field_device(ssg_ctx* ctx, int32_t field, const double* dev, int64_t n);
+int ssg_get_field_device(ssg_ctx* ctx, int32_t field, double* dev, int64_t n);
+
+/* timestep! (src/timestep.jl:267-285) applied nsteps times, including the
+   filter cadence `step % filter_freq == 0`.  Updates state->t, ->step,
+   ->dt_actual.  SSG_RK3_LOWSTORAGE returns SSG_ERR_UNSUPPORTED, as the
+   reference raises (src/timestep.jl:275-277). */
+int ssg_timestep(ssg_ctx* ctx, const ssg_time_params* params, ssg_time_state* state, int64_t nsteps);
+
+/* compute_tendency! (src/nonlinear.jl:22-36) of the current b, φₛ into
+   out (nx*ny).  ab2_reference_aliasing = 1 gives the tendency the AB2 call
+   site produces (src/timestep.jl:159). */
+int ssg_tendency(ssg_ctx* ctx, double* out, int32_t ab2_reference_aliasing);
+/* compute_jacobian! (src/nonlinear.jl:42-57): -J(φₛ, b), not dealiased */
+int ssg_jacobian(ssg_ctx* ctx, double* out);
+
+/* rfft_2d! / irfft_2d! (src/transforms.jl:42-56) in the reference layout;
+   spec has 2*nx*(ny/2+1) doubles */
+int ssg_rfft_2d(ssg_ctx* ctx, const double* real_in, double* spec_out);
+int ssg_irfft_2d(ssg_ctx* ctx, const double* spec_in, double* real_out);
+
+/* apply_spectral_filter! on b (src/timestep.jl:247-260 with
+   ExponentialFilter(strength, order, cutoff), src/filter.jl:45-56) */
+int ssg_apply_exponential_filter(ssg_ctx* ctx, double strength, int32_t order, double cutoff);
+
+int ssg_diagnostics(ssg_ctx* ctx, ssg_diag* out);
+
+/* Kernel timing of the last ssg_timestep call (HIP events on the context
+   stream): total milliseconds and the number of launches. */
+int ssg_last_timing(ssg_ctx* ctx, double* ms, int64_t* launches);
+
+/* Per-launch HIP event timing for roofline reporting.  ssg_profile(1)
+   starts recording an event pair around every kernel launch (and clears
+   earlier records); ssg_profile_read sums, for one kernel class, the
+   measured milliseconds, the launch count and the algorithmic bytes (every
+   input read once, every output written once).  Classes: 0 = row pass
+   forward (k_rows_fwd), 1 = column pass (k_cols), 2 = row pass inverse with
+   fused stage epilogue (k_rows_inv). */
+int ssg_profile(ssg_ctx* ctx, int32_t enable);
+int ssg_profile_read(ssg_ctx* ctx, int32_t kind, double* total_ms, int64_t* count, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SSG_H */

@@ -1,0 +1,173 @@
+// Device-side building blocks: complex arithmetic, small in-register DFTs and
+// a Stockham line FFT that keeps each thread's elements in registers and
+// exchanges them through LDS between stages.
+//
+// Register layout (invariant across a whole line FFT): a line of N complex
+// points is owned by T = N/E consecutive threads; thread t holds elements
+// t + T*m, m = 0..E-1.  A forward or inverse transform maps that layout to
+// itself in natural order, so a symbol multiply can sit between a forward and
+// an inverse transform without touching memory.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ssg {
+
+template <typename R> struct cpx_t;
+template <> struct cpx_t<double> { using type = double2; };
+template <> struct cpx_t<float> { using type = float2; };
+template <typename R> using cpx = typename cpx_t<R>::type;
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+    return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename C> __device__ __forceinline__ C cconj(C a) { return C{a.x, -a.y}; }
+template <typename C, typename R> __device__ __forceinline__ C cscale(C a, R s) { return C{a.x * s, a.y * s}; }
+// multiply by i*s (s real)
+template <typename C, typename R> __device__ __forceinline__ C cmul_i(C a, R s) { return C{-a.y * s, a.x * s}; }
+
+// cos(2*pi*m/16), m taken mod 16
+__device__ __forceinline__ constexpr double cos16(int m) {
+    switch (m & 15) {
+        case 0: return 1.0;
+        case 1: return 0.92387953251128675613;
+        case 2: return 0.70710678118654752440;
+        case 3: return 0.38268343236508977173;
+        case 4: return 0.0;
+        case 5: return -0.38268343236508977173;
+        case 6: return -0.70710678118654752440;
+        case 7: return -0.92387953251128675613;
+        case 8: return -1.0;
+        case 9: return -0.92387953251128675613;
+        case 10: return -0.70710678118654752440;
+        case 11: return -0.38268343236508977173;
+        case 12: return 0.0;
+        case 13: return 0.38268343236508977173;
+        case 14: return 0.70710678118654752440;
+        default: return 0.92387953251128675613;
+    }
+}
+__device__ __forceinline__ constexpr double sin16(int m) { return cos16(m - 4); }
+
+// o * exp(DIR * 2*pi*i * m/16); m is a compile-time constant after unrolling
+template <typename R, int DIR>
+__device__ __forceinline__ cpx<R> tw16(cpx<R> o, int m) {
+    using C = cpx<R>;
+    m &= 15;
+    if (m == 0) return o;
+    if (m == 8) return C{-o.x, -o.y};
+    if (m == 4) return DIR < 0 ? C{o.y, -o.x} : C{-o.y, o.x};
+    if (m == 12) return DIR < 0 ? C{-o.y, o.x} : C{o.y, -o.x};
+    const R c = (R)cos16(m);
+    const R s = (R)(DIR * sin16(m));
+    return C{o.x * c - o.y * s, o.x * s + o.y * c};
+}
+
+// In-register DFT of length L (power of two, L <= 16), natural order in/out.
+template <typename R, int L, int DIR> struct SmallDFT {
+    __device__ __forceinline__ static void run(cpx<R>* v) {
+        cpx<R> e[L / 2], o[L / 2];
+#pragma unroll
+        for (int i = 0; i < L / 2; ++i) {
+            e[i] = v[2 * i];
+            o[i] = v[2 * i + 1];
+        }
+        SmallDFT<R, L / 2, DIR>::run(e);
+        SmallDFT<R, L / 2, DIR>::run(o);
+#pragma unroll
+        for (int k = 0; k < L / 2; ++k) {
+            cpx<R> t = tw16<R, DIR>(o[k], k * (16 / L));
+            v[k] = cadd(e[k], t);
+            v[k + L / 2] = csub(e[k], t);
+        }
+    }
+};
+template <typename R, int DIR> struct SmallDFT<R, 1, DIR> {
+    __device__ __forceinline__ static void run(cpx<R>*) {}
+};
+
+__host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
+
+// LDS index padding: one spare slot every 16 entries keeps the stride-E
+// writes of the first stage off a single bank group.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lds_line_elems(int n) { return n + n / 16 + 1; }
+
+// Stockham line FFT.  v[] holds x[t + T*m] on entry and X[t + T*m] on exit.
+// `buf` is this line's LDS buffer (lds_line_elems(N) entries).  tw[m] =
+// exp(-2*pi*i*m/N), m < N; the inverse uses its conjugate.  Every thread of
+// the block must call this (it contains block barriers).
+template <typename R, int N, int E> struct LineFFT {
+    static constexpr int T = N / E;
+    using C = cpx<R>;
+
+    template <int DIR, int NS>
+    __device__ __forceinline__ static void stage(C* v, C* buf, int t, const C* __restrict__ tw) {
+        constexpr int RAD = cmin(E, N / NS);
+        constexpr int Q = E / RAD;
+        constexpr bool LAST = (NS * RAD == N);
+        C u[E];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int j = t + T * q;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) u[q * RAD + r] = buf[lpad(j + r * (N / RAD))];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int j = t + T * q;
+            const int k = j % NS;
+            C* w = u + q * RAD;
+#pragma unroll
+            for (int r = 1; r < RAD; ++r) {
+                C f = tw[(r * k) * (N / (NS * RAD))];
+                if (DIR > 0) f.y = -f.y;
+                w[r] = cmul(w[r], f);
+            }
+            SmallDFT<R, RAD, DIR>::run(w);
+            if (LAST) {
+#pragma unroll
+                for (int r = 0; r < RAD; ++r) v[q + r * Q] = w[r];
+            } else {
+                const int base = (j / NS) * NS * RAD + k;
+#pragma unroll
+                for (int r = 0; r < RAD; ++r) buf[lpad(base + r * NS)] = w[r];
+            }
+        }
+        if constexpr (!LAST) stage<DIR, NS * RAD>(v, buf, t, tw);
+    }
+
+    // Transform from registers.
+    template <int DIR>
+    __device__ __forceinline__ static void run(C* v, C* buf, int t, const C* __restrict__ tw) {
+        SmallDFT<R, E, DIR>::run(v);
+        if constexpr (T > 1) {
+            __syncthreads();  // earlier readers of buf are done
+#pragma unroll
+            for (int r = 0; r < E; ++r) buf[lpad(t * E + r)] = v[r];
+            stage<DIR, E>(v, buf, t, tw);
+        }
+    }
+
+    // Transform whose input sits in LDS in natural order (buf[lpad(i)] = x[i]).
+    template <int DIR>
+    __device__ __forceinline__ static void run_from_lds(C* v, C* buf, int t, const C* __restrict__ tw) {
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = buf[lpad(t + T * m)];
+        run<DIR>(v, buf, t, tw);
+    }
+
+    // Leave the result in LDS in natural order.
+    __device__ __forceinline__ static void to_lds(const C* v, C* buf, int t) {
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) buf[lpad(t + T * m)] = v[m];
+        __syncthreads();
+    }
+};
+
+}  // namespace ssg

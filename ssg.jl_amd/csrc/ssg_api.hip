@@ -1,0 +1,826 @@
+// C ABI of the SSG time-step library: context, field transfer, the step
+// driver and the spectral sub-boundary.  Declarations: include/ssg.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/ssg.h"
+#include "ssg_kernels.hpp"
+
+using namespace ssg;
+
+namespace {
+
+constexpr int kMaxLdsBytes = 64 * 1024;
+
+template <typename R>
+struct PathBuffers {
+    cpx<R>* w = nullptr;    // (u, v)
+    R* k1 = nullptr;
+    R* fold = nullptr;
+    R* tmp = nullptr;       // real scratch
+    cpx<R>* Fx = nullptr;   // planes, (nx/2+1) * ny
+    cpx<R>* Fy = nullptr;
+    cpx<R>* Tp = nullptr;
+    cpx<R>* twx = nullptr;
+    cpx<R>* twy = nullptr;
+};
+
+}  // namespace
+
+struct ssg_ctx {
+    int nx = 0, ny = 0, nz = 0;
+    double lx = 0, ly = 0, lz = 0;
+    int precision = SSG_FP64;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Grid g{};
+    int ncols_full = 0;   // nx/2 + 1
+    int ncols_tend = 0;   // K + 1
+    double* b = nullptr;
+    double* phi = nullptr;
+    double* dscratch = nullptr;   // nx*ny FP64
+    double2* spec = nullptr;      // nx*(ny/2+1) reference-layout spectrum
+    double* partial = nullptr;    // reduction partials
+    double* partial_host = nullptr;
+    int npartial = 1024;
+    PathBuffers<double> d;
+    PathBuffers<float> f;
+    bool ab2_have_history = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int64_t last_launches = 0;
+    int64_t launches = 0;
+    // per-launch HIP event timing (ssg_profile)
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct ProfRec {
+        int kind;
+        double bytes;
+        int e0, e1;
+    };
+    std::vector<ProfRec> prof_recs;
+    char err[512] = {0};
+};
+
+namespace {
+
+int set_err(ssg_ctx* c, int code, const char* fmt, ...) {
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof(c->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIP_OK(ctx, call)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) return set_err(ctx, SSG_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+bool pow2_in_range(int n) { return n >= 8 && n <= 4096 && (n & (n - 1)) == 0; }
+
+// run f(std::integral_constant<int, N>) for the power of two n
+template <typename Fn>
+int with_size(int n, Fn&& fn) {
+    switch (n) {
+        case 8: return fn(std::integral_constant<int, 8>{});
+        case 16: return fn(std::integral_constant<int, 16>{});
+        case 32: return fn(std::integral_constant<int, 32>{});
+        case 64: return fn(std::integral_constant<int, 64>{});
+        case 128: return fn(std::integral_constant<int, 128>{});
+        case 256: return fn(std::integral_constant<int, 256>{});
+        case 512: return fn(std::integral_constant<int, 512>{});
+        case 1024: return fn(std::integral_constant<int, 1024>{});
+        case 2048: return fn(std::integral_constant<int, 2048>{});
+        case 4096: return fn(std::integral_constant<int, 4096>{});
+        default: return SSG_ERR_UNSUPPORTED;
+    }
+}
+
+constexpr int elems_per_thread(int n) { return n < 16 ? n : 16; }
+
+// lines per block: at most 256 threads and kMaxLdsBytes of LDS, a power of
+// two that divides `rows`
+template <typename R>
+int lines_per_block(int n, int rows) {
+    const int T = n / elems_per_thread(n);
+    const int le = lds_line_elems(n) * (int)sizeof(cpx<R>);
+    int G = 1;
+    while (2 * G * T <= 256 && 2 * G * le <= kMaxLdsBytes && rows % (2 * G) == 0) G *= 2;
+    return G;
+}
+
+int prof_begin(ssg_ctx* c, int* slot);
+int prof_end(ssg_ctx* c, int slot, int kind, double bytes);
+
+template <typename R>
+PathBuffers<R>& bufs(ssg_ctx* c);
+template <>
+PathBuffers<double>& bufs<double>(ssg_ctx* c) { return c->d; }
+template <>
+PathBuffers<float>& bufs<float>(ssg_ctx* c) { return c->f; }
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename R, typename S, int SRC>
+int launch_rows_fwd(ssg_ctx* c, const S* f, const cpx<R>* w, cpx<R>* PA, cpx<R>* PB, int ncols) {
+    PathBuffers<R>& B = bufs<R>(c);
+    return with_size(c->nx, [&](auto tag) -> int {
+        constexpr int N = decltype(tag)::value;
+        constexpr int E = elems_per_thread(N);
+        constexpr int T = N / E;
+        const int rows_per_line = SRC == SRC_FLUX ? 1 : 2;
+        const int G = lines_per_block<R>(N, c->ny / rows_per_line);
+        const int nblk = c->ny / (rows_per_line * G);
+        RowsFwdArgs<R, S> a{f, w, PA, PB, ncols, G, nblk, B.twx, c->g};
+        const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<R>);
+        const double n2 = (double)c->nx * c->ny, plane = (double)ncols * c->ny * sizeof(cpx<R>);
+        const double bytes = SRC == SRC_FLUX ? n2 * (sizeof(double) + sizeof(cpx<R>)) + 2 * plane
+                                             : n2 * sizeof(S) + plane;
+        int slot, rc;
+        if ((rc = prof_begin(c, &slot))) return rc;
+        hipLaunchKernelGGL((k_rows_fwd<R, S, N, E, SRC>), dim3(nblk), dim3(G * T), lds, c->stream, a);
+        c->launches++;
+        HIP_OK(c, hipGetLastError());
+        return prof_end(c, slot, 0, bytes);
+    });
+}
+
+template <typename R, int OP>
+int launch_cols(ssg_ctx* c, const cpx<R>* inA, const cpx<R>* inB, cpx<R>* outA, cpx<R>* outB, int ncols,
+                R scale, R fstrength = 0, int forder = 1, R fcutoff = 1, int fdelta = 0) {
+    PathBuffers<R>& B = bufs<R>(c);
+    return with_size(c->ny, [&](auto tag) -> int {
+        constexpr int M = decltype(tag)::value;
+        constexpr int E = elems_per_thread(M);
+        constexpr int T = M / E;
+        const int le = lds_line_elems(M) * (int)sizeof(cpx<R>);
+        int G = 1;
+        while (2 * G * T <= 256 && 2 * G * le <= kMaxLdsBytes && 2 * G <= ncols) G *= 2;
+        const int nblk = (ncols + G - 1) / G;
+        const double kmx = M_PI * c->nx / c->lx, kmy = M_PI * c->ny / c->ly;
+        ColsArgs<R> a{inA, inB, outA, outB, ncols, G, scale, fstrength, fcutoff,
+                      (R)(1.0 / std::sqrt(kmx * kmx + kmy * kmy)), forder, fdelta, B.twy, c->g};
+        const size_t lds = (size_t)G * le;
+        const int nin = (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC) ? 2 : 1;
+        const int nout = OP == OP_VEL ? 2 : 1;
+        const double bytes = (double)(nin + nout) * ncols * c->ny * sizeof(cpx<R>);
+        int slot, rc;
+        if ((rc = prof_begin(c, &slot))) return rc;
+        hipLaunchKernelGGL((k_cols<R, M, E, OP>), dim3(nblk), dim3(G * T), lds, c->stream, a);
+        c->launches++;
+        HIP_OK(c, hipGetLastError());
+        return prof_end(c, slot, 1, bytes);
+    });
+}
+
+template <typename R, int MODE, int EPI>
+int launch_rows_inv(ssg_ctx* c, RowsInvArgs<R> a) {
+    PathBuffers<R>& B = bufs<R>(c);
+    return with_size(c->nx, [&](auto tag) -> int {
+        constexpr int N = decltype(tag)::value;
+        constexpr int E = elems_per_thread(N);
+        constexpr int T = N / E;
+        const int rows_per_line = MODE == MODE_PAIR ? 2 : 1;
+        const int G = lines_per_block<R>(N, c->ny / rows_per_line);
+        const int nblk = c->ny / (rows_per_line * G);
+        a.G = G;
+        a.nblk = nblk;
+        a.twx = B.twx;
+        a.g = c->g;
+        const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<R>);
+        const double n2 = (double)c->nx * c->ny, sc = sizeof(cpx<R>);
+        double bytes = (MODE == MODE_PAIR ? 1.0 : 2.0) * a.ncols_in * c->ny * sc;
+        if (EPI == EPI_STORE) bytes += n2 * sizeof(R);
+        if (EPI == EPI_STORE_W) bytes += n2 * sc;
+        if (EPI == EPI_RK1) bytes += n2 * (sizeof(double) + sizeof(R));
+        if (EPI == EPI_RK2) bytes += n2 * sizeof(double);
+        if (EPI == EPI_RK3) bytes += n2 * (2 * sizeof(double) + sizeof(R));
+        if (EPI == EPI_AB2) bytes += n2 * (2 * sizeof(double) + 2 * sizeof(R));
+        if (EPI == EPI_SET) bytes += n2 * sizeof(double);
+        if (EPI == EPI_ADD) bytes += n2 * 2 * sizeof(double);
+        if (EPI >= EPI_RK1 && a.emit) bytes += n2 * sc + 2.0 * a.ncols_out * c->ny * sc;
+        int slot, rc;
+        if ((rc = prof_begin(c, &slot))) return rc;
+        hipLaunchKernelGGL((k_rows_inv<R, N, E, MODE, EPI>), dim3(nblk), dim3(G * T), lds, c->stream, a);
+        c->launches++;
+        HIP_OK(c, hipGetLastError());
+        return prof_end(c, slot, 2, bytes);
+    });
+}
+
+// Event pair around one launch when profiling is on.  `bytes` is the
+// launch's algorithmic traffic: every input read once, every output
+// written once.
+int prof_begin(ssg_ctx* c, int* slot) {
+    *slot = -1;
+    if (!c->prof) return SSG_OK;
+    const int need = (int)c->prof_recs.size() * 2 + 2;
+    while ((int)c->ev_pool.size() < need) {
+        hipEvent_t e;
+        HIP_OK(c, hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+    }
+    *slot = (int)c->prof_recs.size() * 2;
+    HIP_OK(c, hipEventRecord(c->ev_pool[*slot], c->stream));
+    return SSG_OK;
+}
+
+int prof_end(ssg_ctx* c, int slot, int kind, double bytes) {
+    if (slot < 0) return SSG_OK;
+    HIP_OK(c, hipEventRecord(c->ev_pool[slot + 1], c->stream));
+    c->prof_recs.push_back({kind, bytes, slot, slot + 1});
+    return SSG_OK;
+}
+
+int grid1d(long n) {
+    long b = (n + 255) / 256;
+    if (b > 4096) b = 4096;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+// ---------------------------------------------------------------------------
+// path pieces
+// ---------------------------------------------------------------------------
+
+// surface velocities (src/setICs.jl:161-192) in FP64, and their FP32 copy
+int compute_velocities(ssg_ctx* c) {
+    PathBuffers<double>& D = c->d;
+    int rc = launch_rows_fwd<double, double, SRC_PAIR>(c, c->phi, nullptr, D.Fx, nullptr, c->ncols_full);
+    if (rc) return rc;
+    rc = launch_cols<double, OP_VEL>(c, D.Fx, nullptr, D.Tp, D.Fy, c->ncols_full, 1.0 / ((double)c->nx * c->ny));
+    if (rc) return rc;
+    RowsInvArgs<double> a{};
+    a.P0 = D.Tp;
+    a.P1 = D.Fy;
+    a.ncols_in = c->ncols_full;
+    a.w_out = D.w;
+    rc = launch_rows_inv<double, MODE_PLANES, EPI_STORE_W>(c, a);
+    if (rc) return rc;
+    if (c->precision == SSG_FP32_MIXED) {
+        const long n = (long)c->nx * c->ny;
+        hipLaunchKernelGGL((k_w_convert<float>), dim3(grid1d(n)), dim3(256), 0, c->stream, D.w, c->f.w, n);
+        c->launches++;
+        HIP_OK(c, hipGetLastError());
+    }
+    return SSG_OK;
+}
+
+// fluxes of the current b into (Fx, Fy) planes
+template <typename R>
+int flux_forward(ssg_ctx* c, int ncols) {
+    PathBuffers<R>& B = bufs<R>(c);
+    return launch_rows_fwd<R, double, SRC_FLUX>(c, c->b, B.w, B.Fx, B.Fy, ncols);
+}
+
+template <typename R, int EPI>
+int stage_inverse(ssg_ctx* c, double c1, int emit, int first) {
+    PathBuffers<R>& B = bufs<R>(c);
+    RowsInvArgs<R> a{};
+    a.P0 = B.Tp;
+    a.ncols_in = c->ncols_tend;
+    a.b = c->b;
+    a.k1 = B.k1;
+    a.fold = B.fold;
+    a.c1 = c1;
+    a.first = first;
+    a.emit = emit;
+    a.w = B.w;
+    a.Fx = B.Fx;
+    a.Fy = B.Fy;
+    a.ncols_out = c->ncols_tend;
+    return launch_rows_inv<R, MODE_PAIR, EPI>(c, a);
+}
+
+template <typename R>
+int tendency_cols(ssg_ctx* c, int quirk) {
+    PathBuffers<R>& B = bufs<R>(c);
+    const R scale = (R)(1.0 / ((double)c->nx * c->ny));
+    if (quirk) return launch_cols<R, OP_TEND_Q>(c, B.Fx, B.Fy, B.Tp, nullptr, c->ncols_tend, scale);
+    return launch_cols<R, OP_TEND>(c, B.Fx, B.Fy, B.Tp, nullptr, c->ncols_tend, scale);
+}
+
+// apply_spectral_filter! on b; FP64 sets b = irfft(σ b̂); FP32-mixed adds
+// the FP32 defect irfft((σ-1) b̂) to the FP64 b.  emit: refresh fluxes.
+template <typename R>
+int filter_b(ssg_ctx* c, double strength, int order, double cutoff, int emit) {
+    PathBuffers<R>& B = bufs<R>(c);
+    int rc = launch_rows_fwd<R, double, SRC_PAIR>(c, c->b, nullptr, B.Tp, nullptr, c->ncols_full);
+    if (rc) return rc;
+    const int delta = std::is_same<R, float>::value ? 1 : 0;
+    rc = launch_cols<R, OP_FILT>(c, B.Tp, nullptr, B.Tp, nullptr, c->ncols_full, (R)(1.0 / ((double)c->nx * c->ny)),
+                                 (R)strength, order, (R)cutoff, delta);
+    if (rc) return rc;
+    RowsInvArgs<R> a{};
+    a.P0 = B.Tp;
+    a.ncols_in = c->ncols_full;
+    a.b = c->b;
+    a.emit = emit;
+    a.w = B.w;
+    a.Fx = B.Fx;
+    a.Fy = B.Fy;
+    a.ncols_out = c->ncols_tend;
+    if (delta) return launch_rows_inv<R, MODE_PAIR, EPI_ADD>(c, a);
+    return launch_rows_inv<R, MODE_PAIR, EPI_SET>(c, a);
+}
+
+template <typename R>
+int run_steps(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, int64_t nsteps) {
+    int rc = flux_forward<R>(c, c->ncols_tend);
+    if (rc) return rc;
+    const double dt = p->dt;
+    for (int64_t it = 0; it < nsteps; ++it) {
+        const int64_t step_after = s->step + 1;
+        const bool filt = p->filter_freq > 0 && (step_after % p->filter_freq) == 0;
+        const bool last = (it == nsteps - 1);
+        const int emit_final = (!filt && !last) ? 1 : 0;
+        if (p->scheme == SSG_RK3) {
+            // src/timestep.jl:195-242
+            if ((rc = tendency_cols<R>(c, 0))) return rc;
+            if ((rc = stage_inverse<R, EPI_RK1>(c, dt / 3.0, 1, 0))) return rc;
+            if ((rc = tendency_cols<R>(c, 0))) return rc;
+            if ((rc = stage_inverse<R, EPI_RK2>(c, 2.0 * dt / 3.0, 1, 0))) return rc;
+            if ((rc = tendency_cols<R>(c, 0))) return rc;
+            if ((rc = stage_inverse<R, EPI_RK3>(c, dt / 4.0, emit_final, 0))) return rc;
+        } else {
+            // src/timestep.jl:152-190
+            if ((rc = tendency_cols<R>(c, p->ab2_reference_aliasing))) return rc;
+            const int first = (s->step == 0) ? 1 : 0;
+            if ((rc = stage_inverse<R, EPI_AB2>(c, dt, emit_final, first))) return rc;
+            c->ab2_have_history = true;
+        }
+        s->t += dt;
+        s->step = step_after;
+        s->dt_actual = dt;
+        if (filt) {
+            // src/timestep.jl:280-281
+            if ((rc = filter_b<R>(c, p->filter_strength, p->filter_order, p->filter_cutoff, last ? 0 : 1))) return rc;
+        }
+    }
+    return SSG_OK;
+}
+
+int check_domain(const ssg_domain_desc* d, char* msg, size_t len) {
+    if (!d) return snprintf(msg, len, "null domain descriptor"), SSG_ERR_ARG;
+    if (!pow2_in_range(d->nx) || !pow2_in_range(d->ny))
+        return snprintf(msg, len, "nx, ny must be powers of two in [8, 4096] (got %d, %d)", d->nx, d->ny),
+               SSG_ERR_UNSUPPORTED;
+    if (d->nz < 1) return snprintf(msg, len, "nz must be >= 1"), SSG_ERR_ARG;
+    if (!(d->lx > 0) || !(d->ly > 0) || !(d->lz > 0)) return snprintf(msg, len, "domain lengths must be > 0"), SSG_ERR_ARG;
+    if (d->precision != SSG_FP64 && d->precision != SSG_FP32_MIXED)
+        return snprintf(msg, len, "unknown precision %d", d->precision), SSG_ERR_ARG;
+    return SSG_OK;
+}
+
+template <typename R>
+int alloc_path(ssg_ctx* c, PathBuffers<R>& B) {
+    const size_t n = (size_t)c->nx * c->ny;
+    const size_t planes = (size_t)c->ncols_full * c->g.pitch;
+    HIP_OK(c, hipMalloc(&B.w, n * sizeof(cpx<R>)));
+    HIP_OK(c, hipMalloc(&B.k1, n * sizeof(R)));
+    HIP_OK(c, hipMalloc(&B.fold, n * sizeof(R)));
+    HIP_OK(c, hipMalloc(&B.tmp, n * sizeof(R)));
+    HIP_OK(c, hipMalloc(&B.Fx, planes * sizeof(cpx<R>)));
+    HIP_OK(c, hipMalloc(&B.Fy, planes * sizeof(cpx<R>)));
+    HIP_OK(c, hipMalloc(&B.Tp, planes * sizeof(cpx<R>)));
+    HIP_OK(c, hipMemsetAsync(B.w, 0, n * sizeof(cpx<R>), c->stream));
+    HIP_OK(c, hipMemsetAsync(B.k1, 0, n * sizeof(R), c->stream));
+    HIP_OK(c, hipMemsetAsync(B.fold, 0, n * sizeof(R), c->stream));
+    HIP_OK(c, hipMemsetAsync(B.Fx, 0, planes * sizeof(cpx<R>), c->stream));
+    HIP_OK(c, hipMemsetAsync(B.Fy, 0, planes * sizeof(cpx<R>), c->stream));
+    HIP_OK(c, hipMemsetAsync(B.Tp, 0, planes * sizeof(cpx<R>), c->stream));
+    // twiddles exp(-2 pi i m / n), computed in FP64
+    for (int axis = 0; axis < 2; ++axis) {
+        const int n1 = axis == 0 ? c->nx : c->ny;
+        std::vector<cpx<R>> h(n1);
+        for (int m = 0; m < n1; ++m) {
+            const double a = -2.0 * M_PI * (double)m / (double)n1;
+            h[m] = cpx<R>{(R)std::cos(a), (R)std::sin(a)};
+        }
+        cpx<R>** dst = axis == 0 ? &B.twx : &B.twy;
+        HIP_OK(c, hipMalloc(dst, n1 * sizeof(cpx<R>)));
+        HIP_OK(c, hipMemcpy(*dst, h.data(), n1 * sizeof(cpx<R>), hipMemcpyHostToDevice));
+    }
+    return SSG_OK;
+}
+
+template <typename R>
+void free_path(PathBuffers<R>& B) {
+    (void)hipFree(B.w);
+    (void)hipFree(B.k1);
+    (void)hipFree(B.fold);
+    (void)hipFree(B.tmp);
+    (void)hipFree(B.Fx);
+    (void)hipFree(B.Fy);
+    (void)hipFree(B.Tp);
+    (void)hipFree(B.twx);
+    (void)hipFree(B.twy);
+    B = PathBuffers<R>{};
+}
+
+int finite_check(ssg_ctx* c, int64_t* nonfinite, double* sum, double* sum2) {
+    const long n = (long)c->nx * c->ny;
+    const int nb = c->npartial / 3;
+    hipLaunchKernelGGL(k_stats, dim3(nb), dim3(256), 0, c->stream, c->b, n, c->partial);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    HIP_OK(c, hipMemcpyAsync(c->partial_host, c->partial, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    double s = 0, s2 = 0, nf = 0;
+    for (int i = 0; i < nb; ++i) {
+        s += c->partial_host[3 * i];
+        s2 += c->partial_host[3 * i + 1];
+        nf += c->partial_host[3 * i + 2];
+    }
+    if (nonfinite) *nonfinite = (int64_t)nf;
+    if (sum) *sum = s;
+    if (sum2) *sum2 = s2;
+    return SSG_OK;
+}
+
+// rfft2 of a device FP64 field into c->d.Tp (this library's layout, FP64)
+int rfft2_device(ssg_ctx* c, const double* src) {
+    PathBuffers<double>& D = c->d;
+    int rc = launch_rows_fwd<double, double, SRC_PAIR>(c, src, nullptr, D.Tp, nullptr, c->ncols_full);
+    if (rc) return rc;
+    return launch_cols<double, OP_FWD>(c, D.Tp, nullptr, D.Tp, nullptr, c->ncols_full, 1.0);
+}
+
+double parseval_of_tp(ssg_ctx* c, int* rc_out) {
+    const int nb = c->npartial;
+    hipLaunchKernelGGL((k_parseval_ref<double>), dim3(nb), dim3(256), 0, c->stream, c->d.Tp, c->g, c->partial);
+    c->launches++;
+    *rc_out = SSG_OK;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(c->partial_host, c->partial, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        *rc_out = set_err(c, SSG_ERR_HIP, "parseval reduction failed");
+        return 0.0;
+    }
+    double s = 0;
+    for (int i = 0; i < nb; ++i) s += c->partial_host[i];
+    // src/transforms.jl:271: Lx*Ly*Lz / (Nx^2 Ny^2 Nz)
+    return s * (c->lx * c->ly * c->lz) / ((double)c->nx * c->nx * (double)c->ny * c->ny * (double)c->nz);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int ssg_create(ssg_ctx** out, const ssg_domain_desc* desc) {
+    if (!out) return SSG_ERR_ARG;
+    *out = nullptr;
+    char msg[256];
+    int rc = check_domain(desc, msg, sizeof(msg));
+    if (rc) {
+        fprintf(stderr, "ssg_create: %s\n", msg);
+        return rc;
+    }
+    ssg_ctx* c = new (std::nothrow) ssg_ctx();
+    if (!c) return SSG_ERR_ARG;
+    c->nx = desc->nx;
+    c->ny = desc->ny;
+    c->nz = desc->nz;
+    c->lx = desc->lx;
+    c->ly = desc->ly;
+    c->lz = desc->lz;
+    c->precision = desc->precision;
+    c->device = desc->device;
+    c->g.nx = c->nx;
+    c->g.ny = c->ny;
+    c->g.pitch = c->ny;
+    c->g.K = c->nx / 3;
+    c->g.Ky = 2 * (c->ny / 2 + 1) / 3;
+    c->g.dkx = 2.0 * M_PI / c->lx;
+    c->g.dky = 2.0 * M_PI / c->ly;
+    c->ncols_full = c->nx / 2 + 1;
+    c->ncols_tend = c->g.K + 1;
+    auto fail = [&](int code) {
+        ssg_destroy(c);
+        return code;
+    };
+    if (hipSetDevice(c->device) != hipSuccess) {
+        fprintf(stderr, "ssg_create: hipSetDevice(%d) failed\n", c->device);
+        return fail(SSG_ERR_HIP);
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(SSG_ERR_HIP);
+    const size_t n = (size_t)c->nx * c->ny;
+    const size_t nspec = (size_t)c->nx * (c->ny / 2 + 1);
+    if (hipMalloc(&c->b, n * sizeof(double)) != hipSuccess || hipMalloc(&c->phi, n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->dscratch, n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->spec, nspec * sizeof(double2)) != hipSuccess ||
+        hipMalloc(&c->partial, c->npartial * 3 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&c->partial_host, c->npartial * 3 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return fail(SSG_ERR_HIP);
+    (void)hipMemsetAsync(c->b, 0, n * sizeof(double), c->stream);
+    (void)hipMemsetAsync(c->phi, 0, n * sizeof(double), c->stream);
+    if ((rc = alloc_path<double>(c, c->d))) {
+        fprintf(stderr, "ssg_create: %s\n", c->err);
+        return fail(rc);
+    }
+    if (c->precision == SSG_FP32_MIXED && (rc = alloc_path<float>(c, c->f))) {
+        fprintf(stderr, "ssg_create: %s\n", c->err);
+        return fail(rc);
+    }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(SSG_ERR_HIP);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(SSG_ERR_HIP);
+    *out = c;
+    return SSG_OK;
+}
+
+int ssg_destroy(ssg_ctx* c) {
+    if (!c) return SSG_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_path(c->d);
+    free_path(c->f);
+    (void)hipFree(c->b);
+    (void)hipFree(c->phi);
+    (void)hipFree(c->dscratch);
+    (void)hipFree(c->spec);
+    (void)hipFree(c->partial);
+    if (c->partial_host) (void)hipHostFree(c->partial_host);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SSG_OK;
+}
+
+const char* ssg_last_error(const ssg_ctx* c) { return c ? c->err : "null context"; }
+
+int ssg_sync(ssg_ctx* c) {
+    if (!c) return SSG_ERR_ARG;
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+static int field_io(ssg_ctx* c, int32_t field, const double* src, double* dst, int64_t n, bool device) {
+    if (!c) return SSG_ERR_ARG;
+    const int64_t need = (int64_t)c->nx * c->ny;
+    if (n != need) return set_err(c, SSG_ERR_ARG, "field size %lld != nx*ny = %lld", (long long)n, (long long)need);
+    HIP_OK(c, hipSetDevice(c->device));
+    const size_t bytes = (size_t)need * sizeof(double);
+    if (src) {
+        if (!src) return set_err(c, SSG_ERR_ARG, "null pointer");
+        const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        if (field == SSG_FIELD_B) {
+            HIP_OK(c, hipMemcpyAsync(c->b, src, bytes, kind, c->stream));
+            c->ab2_have_history = false;
+        } else if (field == SSG_FIELD_PHI) {
+            HIP_OK(c, hipMemcpyAsync(c->phi, src, bytes, kind, c->stream));
+            int rc = compute_velocities(c);
+            if (rc) return rc;
+        } else {
+            return set_err(c, SSG_ERR_ARG, "field %d cannot be set", field);
+        }
+    } else {
+        if (!dst) return set_err(c, SSG_ERR_ARG, "null pointer");
+        const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        if (field == SSG_FIELD_B) {
+            HIP_OK(c, hipMemcpyAsync(dst, c->b, bytes, kind, c->stream));
+        } else if (field == SSG_FIELD_PHI) {
+            HIP_OK(c, hipMemcpyAsync(dst, c->phi, bytes, kind, c->stream));
+        } else if (field == SSG_FIELD_U || field == SSG_FIELD_V) {
+            const long nn = (long)need;
+            double* u = field == SSG_FIELD_U ? c->dscratch : c->d.tmp;
+            double* v = field == SSG_FIELD_U ? c->d.tmp : c->dscratch;
+            hipLaunchKernelGGL((k_w_split<double>), dim3(grid1d(nn)), dim3(256), 0, c->stream, c->d.w, u, v, nn);
+            HIP_OK(c, hipGetLastError());
+            HIP_OK(c, hipMemcpyAsync(dst, c->dscratch, bytes, kind, c->stream));
+        } else {
+            return set_err(c, SSG_ERR_ARG, "unknown field %d", field);
+        }
+    }
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+int ssg_set_field(ssg_ctx* c, int32_t field, const double* host, int64_t n) {
+    if (!host) return set_err(c, SSG_ERR_ARG, "null pointer");
+    return field_io(c, field, host, nullptr, n, false);
+}
+int ssg_get_field(ssg_ctx* c, int32_t field, double* host, int64_t n) {
+    if (!host) return set_err(c, SSG_ERR_ARG, "null pointer");
+    return field_io(c, field, nullptr, host, n, false);
+}
+int ssg_set_field_device(ssg_ctx* c, int32_t field, const double* dev, int64_t n) {
+    if (!dev) return set_err(c, SSG_ERR_ARG, "null pointer");
+    return field_io(c, field, dev, nullptr, n, true);
+}
+int ssg_get_field_device(ssg_ctx* c, int32_t field, double* dev, int64_t n) {
+    if (!dev) return set_err(c, SSG_ERR_ARG, "null pointer");
+    return field_io(c, field, nullptr, dev, n, true);
+}
+
+int ssg_timestep(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, int64_t nsteps) {
+    if (!c) return SSG_ERR_ARG;
+    if (!p || !s) return set_err(c, SSG_ERR_ARG, "null params/state");
+    if (nsteps < 0) return set_err(c, SSG_ERR_ARG, "nsteps < 0");
+    if (p->scheme == SSG_RK3_LOWSTORAGE)
+        return set_err(c, SSG_ERR_UNSUPPORTED, "Time integration scheme RK3_LowStorage not implemented yet");
+    if (p->scheme != SSG_RK3 && p->scheme != SSG_AB2_LOWSTORAGE)
+        return set_err(c, SSG_ERR_ARG, "unknown scheme %d", p->scheme);
+    if (!(p->dt > 0) || !std::isfinite(p->dt)) return set_err(c, SSG_ERR_ARG, "dt must be finite and > 0");
+    if (p->filter_freq > 0) {
+        // ExponentialFilter constructor checks, src/filter.jl:51-53
+        if (!(p->filter_strength >= 0)) return set_err(c, SSG_ERR_ARG, "Filter strength must be non-negative");
+        if (p->filter_order < 1) return set_err(c, SSG_ERR_ARG, "Filter order must be positive");
+        if (!(p->filter_cutoff > 0 && p->filter_cutoff <= 1))
+            return set_err(c, SSG_ERR_ARG, "Cutoff must be between 0 and 1");
+    }
+    if (p->scheme == SSG_AB2_LOWSTORAGE && s->step > 0 && !c->ab2_have_history)
+        return set_err(c, SSG_ERR_ARG, "AB2 step %lld needs the previous tendency; none is held", (long long)s->step);
+    HIP_OK(c, hipSetDevice(c->device));
+    if (nsteps == 0) return SSG_OK;
+    const int64_t l0 = c->launches;
+    HIP_OK(c, hipEventRecord(c->ev0, c->stream));
+    int rc = c->precision == SSG_FP32_MIXED ? run_steps<float>(c, p, s, nsteps) : run_steps<double>(c, p, s, nsteps);
+    if (rc) return rc;
+    HIP_OK(c, hipEventRecord(c->ev1, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIP_OK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = c->launches - l0;
+    if (p->check_finite) {
+        int64_t nf = 0;
+        if ((rc = finite_check(c, &nf, nullptr, nullptr))) return rc;
+        if (nf) return set_err(c, SSG_ERR_NONFINITE, "%lld non-finite buoyancy values after step %lld", (long long)nf,
+                               (long long)s->step);
+    }
+    return SSG_OK;
+}
+
+int ssg_profile(ssg_ctx* c, int32_t enable) {
+    if (!c) return SSG_ERR_ARG;
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    c->prof = enable != 0;
+    c->prof_recs.clear();
+    return SSG_OK;
+}
+
+int ssg_profile_read(ssg_ctx* c, int32_t kind, double* total_ms, int64_t* count, double* bytes) {
+    if (!c) return SSG_ERR_ARG;
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    double ms = 0, by = 0;
+    int64_t n = 0;
+    for (const auto& r : c->prof_recs) {
+        if (r.kind != kind) continue;
+        float t = 0;
+        HIP_OK(c, hipEventElapsedTime(&t, c->ev_pool[r.e0], c->ev_pool[r.e1]));
+        ms += t;
+        by += r.bytes;
+        ++n;
+    }
+    if (total_ms) *total_ms = ms;
+    if (count) *count = n;
+    if (bytes) *bytes = by;
+    return SSG_OK;
+}
+
+int ssg_last_timing(ssg_ctx* c, double* ms, int64_t* launches) {
+    if (!c) return SSG_ERR_ARG;
+    if (ms) *ms = c->last_ms;
+    if (launches) *launches = c->last_launches;
+    return SSG_OK;
+}
+
+}  // extern "C"
+
+template <typename R>
+static int tendency_impl(ssg_ctx* c, double* out, int quirk, int jac) {
+    PathBuffers<R>& B = bufs<R>(c);
+    const int nc = jac ? c->ncols_full : c->ncols_tend;
+    int rc = flux_forward<R>(c, nc);
+    if (rc) return rc;
+    const R scale = (R)(1.0 / ((double)c->nx * c->ny));
+    if (jac)
+        rc = launch_cols<R, OP_JAC>(c, B.Fx, B.Fy, B.Tp, nullptr, nc, scale);
+    else
+        rc = tendency_cols<R>(c, quirk);
+    if (rc) return rc;
+    RowsInvArgs<R> a{};
+    a.P0 = B.Tp;
+    a.ncols_in = nc;
+    a.out_r = B.tmp;
+    if ((rc = launch_rows_inv<R, MODE_PAIR, EPI_STORE>(c, a))) return rc;
+    const long n = (long)c->nx * c->ny;
+    const double* src = nullptr;
+    if (std::is_same<R, double>::value) {
+        src = reinterpret_cast<const double*>(B.tmp);
+    } else {
+        hipLaunchKernelGGL((k_convert<R, double>), dim3(grid1d(n)), dim3(256), 0, c->stream, B.tmp, c->dscratch, n);
+        HIP_OK(c, hipGetLastError());
+        src = c->dscratch;
+    }
+    HIP_OK(c, hipMemcpyAsync(out, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+extern "C" {
+
+int ssg_tendency(ssg_ctx* c, double* out, int32_t quirk) {
+    if (!c) return SSG_ERR_ARG;
+    if (!out) return set_err(c, SSG_ERR_ARG, "null pointer");
+    HIP_OK(c, hipSetDevice(c->device));
+    return c->precision == SSG_FP32_MIXED ? tendency_impl<float>(c, out, quirk, 0) : tendency_impl<double>(c, out, quirk, 0);
+}
+
+int ssg_jacobian(ssg_ctx* c, double* out) {
+    if (!c) return SSG_ERR_ARG;
+    if (!out) return set_err(c, SSG_ERR_ARG, "null pointer");
+    HIP_OK(c, hipSetDevice(c->device));
+    return c->precision == SSG_FP32_MIXED ? tendency_impl<float>(c, out, 0, 1) : tendency_impl<double>(c, out, 0, 1);
+}
+
+int ssg_rfft_2d(ssg_ctx* c, const double* real_in, double* spec_out) {
+    if (!c) return SSG_ERR_ARG;
+    if (!real_in || !spec_out) return set_err(c, SSG_ERR_ARG, "null pointer");
+    HIP_OK(c, hipSetDevice(c->device));
+    const size_t n = (size_t)c->nx * c->ny, nspec = (size_t)c->nx * (c->ny / 2 + 1);
+    HIP_OK(c, hipMemcpyAsync(c->dscratch, real_in, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = rfft2_device(c, c->dscratch);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_to_ref<double>), dim3(grid1d((long)nspec)), dim3(256), 0, c->stream, c->d.Tp, c->spec, c->g);
+    HIP_OK(c, hipGetLastError());
+    HIP_OK(c, hipMemcpyAsync(spec_out, c->spec, nspec * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+int ssg_irfft_2d(ssg_ctx* c, const double* spec_in, double* real_out) {
+    if (!c) return SSG_ERR_ARG;
+    if (!spec_in || !real_out) return set_err(c, SSG_ERR_ARG, "null pointer");
+    HIP_OK(c, hipSetDevice(c->device));
+    const size_t n = (size_t)c->nx * c->ny, nspec = (size_t)c->nx * (c->ny / 2 + 1);
+    PathBuffers<double>& D = c->d;
+    HIP_OK(c, hipMemcpyAsync(c->spec, spec_in, nspec * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    const long total = (long)c->ncols_full * c->ny;
+    hipLaunchKernelGGL((k_from_ref<double>), dim3(grid1d(total)), dim3(256), 0, c->stream, c->spec, D.Tp, c->g);
+    HIP_OK(c, hipGetLastError());
+    int rc = launch_cols<double, OP_INV>(c, D.Tp, nullptr, D.Tp, nullptr, c->ncols_full, 1.0 / ((double)c->nx * c->ny));
+    if (rc) return rc;
+    RowsInvArgs<double> a{};
+    a.P0 = D.Tp;
+    a.ncols_in = c->ncols_full;
+    a.out_r = D.tmp;
+    if ((rc = launch_rows_inv<double, MODE_PAIR, EPI_STORE>(c, a))) return rc;
+    HIP_OK(c, hipMemcpyAsync(real_out, D.tmp, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+int ssg_apply_exponential_filter(ssg_ctx* c, double strength, int32_t order, double cutoff) {
+    if (!c) return SSG_ERR_ARG;
+    if (!(strength >= 0)) return set_err(c, SSG_ERR_ARG, "Filter strength must be non-negative");
+    if (order < 1) return set_err(c, SSG_ERR_ARG, "Filter order must be positive");
+    if (!(cutoff > 0 && cutoff <= 1)) return set_err(c, SSG_ERR_ARG, "Cutoff must be between 0 and 1");
+    HIP_OK(c, hipSetDevice(c->device));
+    int rc = c->precision == SSG_FP32_MIXED ? filter_b<float>(c, strength, order, cutoff, 0)
+                                            : filter_b<double>(c, strength, order, cutoff, 0);
+    if (rc) return rc;
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+int ssg_diagnostics(ssg_ctx* c, ssg_diag* out) {
+    if (!c) return SSG_ERR_ARG;
+    if (!out) return set_err(c, SSG_ERR_ARG, "null pointer");
+    HIP_OK(c, hipSetDevice(c->device));
+    int rc = finite_check(c, &out->nonfinite, &out->sum_b, &out->sum_b2);
+    if (rc) return rc;
+    if ((rc = rfft2_device(c, c->b))) return rc;
+    out->parseval_b = parseval_of_tp(c, &rc);
+    if (rc) return rc;
+    // surface KE from u, v (src/setICs.jl:225-246): ½ parsevalsum2(û) + ½ parsevalsum2(v̂)
+    const long n = (long)c->nx * c->ny;
+    hipLaunchKernelGGL((k_w_split<double>), dim3(grid1d(n)), dim3(256), 0, c->stream, c->d.w, c->dscratch, c->d.tmp, n);
+    HIP_OK(c, hipGetLastError());
+    if ((rc = rfft2_device(c, c->dscratch))) return rc;
+    const double ku = parseval_of_tp(c, &rc);
+    if (rc) return rc;
+    if ((rc = rfft2_device(c, c->d.tmp))) return rc;
+    const double kv = parseval_of_tp(c, &rc);
+    if (rc) return rc;
+    out->ke_surface = 0.5 * ku + 0.5 * kv;
+    return SSG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,595 @@
+// HIP kernels of the SSG surface time step.
+//
+// Layout in HBM
+//   real 2D fields      : a[x + nx*y]                 (the reference's column-major order)
+//   velocity pair w     : w[x + nx*y] = (u, v)        (one complex-typed load per point)
+//   spectral planes     : P[kx*pitch + y], kx in [0, ncols), y in [0, ny)
+//       after the row pass the y axis is still physical; after the column
+//       pass it holds ky in FFT order.  Only kx <= nx/2 is stored (the real
+//       transform is taken along the contiguous x axis) and only the first
+//       `ncols` columns that the consumer can see are written.
+//
+// Three passes make a spectral operator E[S] f = irfft2(S * rfft2 f):
+//   rows_fwd : per row, x-FFT of two packed real lines, split, store [kx][y]
+//   cols     : per kx column, y-FFT, symbol, inverse y-FFT
+//   rows_inv : per row pair, inverse x-FFT (c2r via two-for-one packing) and a
+//              fused epilogue (stage update, next-stage fluxes + x-FFT)
+// The symbols are the full-plane operators that the reference's r2c-along-y
+// transforms realise (see `sym_*` below), so results match the reference's
+// irfft2(S .* rfft2(f)) chains without reproducing its layout.
+#pragma once
+#include "fft_line.hpp"
+
+namespace ssg {
+
+enum { SRC_FLUX = 0, SRC_PAIR = 1 };
+enum { MODE_PAIR = 0, MODE_PLANES = 1 };
+enum {
+    EPI_STORE = 0,   // out_r[row] = value
+    EPI_STORE_W = 1, // w[row] = (value0, value1)   (MODE_PLANES)
+    EPI_RK1 = 2,
+    EPI_RK2 = 3,
+    EPI_RK3 = 4,
+    EPI_AB2 = 5,
+    EPI_SET = 6,     // b = value        (filter, FP64)
+    EPI_ADD = 7      // b += value       (filter, defect form)
+};
+enum {
+    OP_TEND = 0,     // -M (Dx Fx + Dy Fy)
+    OP_TEND_Q = 1,   // -2 M Dy Fy        (AB2 buffer aliasing of the reference)
+    OP_JAC = 2,      // -(Dx Fx + Dy Fy)  (compute_jacobian!)
+    OP_VEL = 3,      // U = -Dy P, V = Dx P
+    OP_FILT = 4,     // sigma P  (or (sigma-1) P)
+    OP_FWD = 5,      // y-FFT only
+    OP_INV = 6,      // inverse y-FFT only
+    OP_DX = 7,       // Dx P
+    OP_DY = 8,       // Dy P
+    OP_MASK = 9      // M P
+};
+
+struct Grid {
+    int nx, ny;
+    int pitch;      // y pitch of the spectral planes (elements)
+    int K;          // nx / 3
+    int Ky;         // 2 * (ny/2 + 1) / 3
+    double dkx, dky;
+};
+
+// ---------------------------------------------------------------------------
+// Full-plane symbols.  c = kx column (0..nx/2), s = signed ky index in FFT
+// order (row r -> r < ny/2 ? r : r - ny, so the y-Nyquist is -ny/2).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int signed_k(int r, int n) { return r < n / 2 ? r : r - n; }
+
+// ddx_2d! (src/transforms.jl:135-150) through irfft2: i*kx, except on the
+// x-Nyquist column where the reference's kx = -nx/2 meets the Hermitian
+// extension of its y-half-plane: +i*kN for ky > 0, -i*kN for ky < 0, 0 on
+// ky = 0 and on the y-Nyquist row.
+template <typename R>
+__device__ __forceinline__ R sym_dx(const Grid& g, int c, int s) {
+    if (2 * c < g.nx) return (R)(c * g.dkx);
+    const R kn = (R)(-(g.nx / 2) * g.dkx);
+    if (s > 0) return kn;
+    if (s < 0 && s != -(g.ny / 2)) return -kn;
+    return (R)0;
+}
+// ddy_2d! (src/transforms.jl:152-166) through irfft2: i*ky, zero on the
+// y-Nyquist row (its i*ky*f is discarded by the c2r along y).
+template <typename R>
+__device__ __forceinline__ R sym_dy(const Grid& g, int s) {
+    if (2 * s == -g.ny) return (R)0;
+    return (R)(s * g.dky);
+}
+// twothirds_mask (src/domain.jl:498-514) seen through irfft2: kept kx are
+// [-K, K-1] on the stored ky >= 0 half, so ky < 0 keeps kx in [-K+1, K] and
+// the ky = 0 line keeps |kx| < K at weight 1 and |kx| = K at weight 1/2.
+template <typename R>
+__device__ __forceinline__ R sym_mask(const Grid& g, int c, int s) {
+    if (s > 0) return (s <= g.Ky - 1 && c <= g.K - 1) ? (R)1 : (R)0;
+    if (s < 0) return (-s <= g.Ky - 1 && c <= g.K) ? (R)1 : (R)0;
+    if (c <= g.K - 1) return (R)1;
+    return c == g.K ? (R)0.5 : (R)0;
+}
+// exponential_transfer (src/filter.jl:119-126) at k_norm = |k| / kmax
+template <typename R>
+__device__ __forceinline__ R sym_filter(const Grid& g, int c, int s, R strength, int order, R cutoff,
+                                        R inv_kmax) {
+    const R kx = (R)(c * g.dkx), ky = (R)(s * g.dky);
+    const R kn = sqrt(kx * kx + ky * ky) * inv_kmax;
+    if (kn <= cutoff) return (R)1;
+    const R ratio = (kn - cutoff) / ((R)1 - cutoff);
+    R p = (R)1;
+    for (int i = 0; i < order; ++i) p *= ratio;
+    return exp(-strength * p);
+}
+
+// ---------------------------------------------------------------------------
+// Row pass, forward: each line packs two real sequences a + i*b, takes the
+// x-FFT and stores the two half spectra (kx < ncols) to [kx][y] planes.
+//   SRC_FLUX: line l <-> row y0+l, a = u*b, b = v*b  -> (PA, PB) at row y0+l
+//   SRC_PAIR: line l <-> rows (y0+l, y0+G+l) of f   -> PA at both rows
+// ---------------------------------------------------------------------------
+template <typename R, typename S>
+struct RowsFwdArgs {
+    const S* f;            // SRC_PAIR source, or the buoyancy for SRC_FLUX
+    const cpx<R>* w;       // (u, v) for SRC_FLUX
+    cpx<R>* PA;
+    cpx<R>* PB;
+    int ncols;
+    int G;                 // lines per block
+    int nblk;              // grid size (for the XCD-aware block order)
+    const cpx<R>* twx;
+    Grid g;
+};
+
+// Blocks are dealt to the 8 XCDs round robin; renumber them so that the
+// blocks resident on one XCD own neighbouring rows (shared L2 lines).
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+    if ((nblk & 7) != 0) return b;
+    return (b & 7) * (nblk >> 3) + (b >> 3);
+}
+
+template <typename R, int N, int E>
+__device__ __forceinline__ void store_split(cpx<R>* lds, int G, int ncols, cpx<R>* PA, cpx<R>* PB,
+                                            int pitch, int rowA0, int rowB0) {
+    using C = cpx<R>;
+    constexpr int LE = lds_line_elems(N);
+    const int total = ncols * G;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int kx = idx / G, l = idx - kx * G;
+        const C* buf = lds + l * LE;
+        const C z1 = buf[lpad(kx)];
+        const C z2 = cconj(buf[lpad((N - kx) & (N - 1))]);
+        const C a = C{(z1.x + z2.x) * (R)0.5, (z1.y + z2.y) * (R)0.5};
+        // (z1 - z2) / (2i)
+        const C d = C{(z1.y - z2.y) * (R)0.5, -(z1.x - z2.x) * (R)0.5};
+        PA[(size_t)kx * pitch + rowA0 + l] = a;
+        PB[(size_t)kx * pitch + rowB0 + l] = d;
+    }
+}
+
+template <typename R, typename S, int N, int E, int SRC>
+__global__ __launch_bounds__(256) void k_rows_fwd(RowsFwdArgs<R, S> a) {
+    using C = cpx<R>;
+    using F = LineFFT<R, N, E>;
+    constexpr int T = F::T;
+    constexpr int LE = lds_line_elems(N);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int l = threadIdx.x / T, t = threadIdx.x - l * T;
+    const int G = a.G;
+    const int blk = xcd_block(blockIdx.x, a.nblk);
+    const int nx = a.g.nx;
+    C v[E];
+    int rowA0, rowB0;
+    if (SRC == SRC_FLUX) {
+        const int y = blk * G + l;
+        rowA0 = rowB0 = blk * G;
+        const size_t base = (size_t)nx * y;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int x = t + T * m;
+            const R bb = (R)a.f[base + x];
+            const C uv = a.w[base + x];
+            v[m] = C{uv.x * bb, uv.y * bb};
+        }
+    } else {
+        const int yA = blk * 2 * G + l, yB = yA + G;
+        rowA0 = blk * 2 * G;
+        rowB0 = rowA0 + G;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int x = t + T * m;
+            v[m] = C{(R)a.f[(size_t)nx * yA + x], (R)a.f[(size_t)nx * yB + x]};
+        }
+    }
+    C* buf = lds + l * LE;
+    F::template run<-1>(v, buf, t, a.twx);
+    F::to_lds(v, buf, t);
+    store_split<R, N, E>(lds, G, a.ncols, a.PA, SRC == SRC_FLUX ? a.PB : a.PA, a.g.pitch, rowA0, rowB0);
+}
+
+// ---------------------------------------------------------------------------
+// Column pass: per kx column, y-FFT -> symbol -> inverse y-FFT.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct ColsArgs {
+    const cpx<R>* inA;
+    const cpx<R>* inB;
+    cpx<R>* outA;
+    cpx<R>* outB;
+    int ncols;
+    int G;
+    R scale;
+    R fstrength, fcutoff, inv_kmax;
+    int forder;
+    int fdelta;            // filter returns (sigma - 1) * P
+    const cpx<R>* twy;
+    Grid g;
+};
+
+template <typename R, int M, int E, int OP>
+__global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
+    using C = cpx<R>;
+    using F = LineFFT<R, M, E>;
+    constexpr int T = F::T;
+    constexpr int LE = lds_line_elems(M);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int l = threadIdx.x / T, t = threadIdx.x - l * T;
+    const int c = blockIdx.x * a.G + l;
+    const bool live = c < a.ncols;
+    const int pitch = a.g.pitch;
+    C* buf = lds + l * LE;
+    C va[E];
+    const C* srcA = a.inA + (size_t)(live ? c : 0) * pitch;
+#pragma unroll
+    for (int m = 0; m < E; ++m) va[m] = live ? srcA[t + T * m] : C{0, 0};
+
+    if (OP == OP_INV) {
+        F::template run<+1>(va, buf, t, a.twy);
+#pragma unroll
+        for (int m = 0; m < E; ++m) va[m] = cscale(va[m], a.scale);
+        if (live) {
+            C* dst = a.outA + (size_t)c * pitch;
+#pragma unroll
+            for (int m = 0; m < E; ++m) dst[t + T * m] = va[m];
+        }
+        return;
+    }
+    F::template run<-1>(va, buf, t, a.twy);
+    if (OP == OP_FWD) {
+        if (live) {
+            C* dst = a.outA + (size_t)c * pitch;
+#pragma unroll
+            for (int m = 0; m < E; ++m) dst[t + T * m] = va[m];
+        }
+        return;
+    }
+    if (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC) {
+        C vb[E];
+        const C* srcB = a.inB + (size_t)(live ? c : 0) * pitch;
+#pragma unroll
+        for (int m = 0; m < E; ++m) vb[m] = live ? srcB[t + T * m] : C{0, 0};
+        F::template run<-1>(vb, buf, t, a.twy);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int s = signed_k(t + T * m, a.g.ny);
+            const R dy = sym_dy<R>(a.g, s);
+            C r;
+            if (OP == OP_TEND_Q) {
+                r = cmul_i(vb[m], dy * (R)2);
+            } else {
+                const R dx = sym_dx<R>(a.g, c, s);
+                // i*dx*A + i*dy*B
+                r = C{-(dx * va[m].y + dy * vb[m].y), dx * va[m].x + dy * vb[m].x};
+            }
+            R f = -a.scale;
+            if (OP != OP_JAC) f *= sym_mask<R>(a.g, c, s);
+            va[m] = cscale(r, f);
+        }
+    } else if (OP == OP_VEL) {
+        C vb[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int s = signed_k(t + T * m, a.g.ny);
+            const R dy = sym_dy<R>(a.g, s), dx = sym_dx<R>(a.g, c, s);
+            vb[m] = cmul_i(va[m], dx * a.scale);    // V = Dx P
+            va[m] = cmul_i(va[m], -dy * a.scale);   // U = -Dy P
+        }
+        F::template run<+1>(vb, buf, t, a.twy);
+        if (live) {
+            C* dst = a.outB + (size_t)c * pitch;
+#pragma unroll
+            for (int m = 0; m < E; ++m) dst[t + T * m] = vb[m];
+        }
+    } else if (OP == OP_FILT) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int s = signed_k(t + T * m, a.g.ny);
+            R sg = sym_filter<R>(a.g, c, s, a.fstrength, a.forder, a.fcutoff, a.inv_kmax);
+            if (a.fdelta) sg -= (R)1;
+            va[m] = cscale(va[m], sg * a.scale);
+        }
+    } else if (OP == OP_DX || OP == OP_DY || OP == OP_MASK) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int s = signed_k(t + T * m, a.g.ny);
+            if (OP == OP_DX) va[m] = cmul_i(va[m], sym_dx<R>(a.g, c, s) * a.scale);
+            if (OP == OP_DY) va[m] = cmul_i(va[m], sym_dy<R>(a.g, s) * a.scale);
+            if (OP == OP_MASK) va[m] = cscale(va[m], sym_mask<R>(a.g, c, s) * a.scale);
+        }
+    }
+    F::template run<+1>(va, buf, t, a.twy);
+    if (live) {
+        C* dst = a.outA + (size_t)c * pitch;
+#pragma unroll
+        for (int m = 0; m < E; ++m) dst[t + T * m] = va[m];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row pass, inverse + epilogue.
+//   MODE_PAIR  : line l <-> rows (y0+l, y0+G+l) of plane P0
+//   MODE_PLANES: line l <-> row y0+l of planes (P0, P1)
+// The c2r along x ignores the imaginary parts of the kx = 0 and kx = nx/2
+// bins (the same convention the reference applies along y).
+// ---------------------------------------------------------------------------
+template <typename R>
+struct RowsInvArgs {
+    const cpx<R>* P0;
+    const cpx<R>* P1;
+    int ncols_in;
+    int G;
+    int nblk;
+    // epilogue
+    double* b;             // prognostic buoyancy (FP64 in every precision)
+    R* k1;                 // RK3 first-stage tendency
+    R* fold;               // AB2 previous tendency
+    R* out_r;              // EPI_STORE
+    cpx<R>* w_out;         // EPI_STORE_W
+    double c1, c2;         // stage coefficients
+    int first;             // AB2: forward Euler on step 0
+    // next-stage flux emission
+    int emit;
+    const cpx<R>* w;
+    cpx<R>* Fx;
+    cpx<R>* Fy;
+    int ncols_out;
+    const cpx<R>* twx;
+    Grid g;
+};
+
+template <typename R, int N, int E, int MODE, int EPI>
+__global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
+    using C = cpx<R>;
+    using F = LineFFT<R, N, E>;
+    constexpr int T = F::T;
+    constexpr int LE = lds_line_elems(N);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int l = threadIdx.x / T, t = threadIdx.x - l * T;
+    const int G = a.G;
+    const int blk = xcd_block(blockIdx.x, a.nblk);
+    const int pitch = a.g.pitch, nx = a.g.nx;
+    const int rows_per_blk = (MODE == MODE_PAIR) ? 2 * G : G;
+    const int y0 = blk * rows_per_blk;
+    const int rowA = y0 + l;
+    const int rowB = (MODE == MODE_PAIR) ? y0 + G + l : rowA;
+
+    // cooperative gather of the packed spectra into the per-line buffers
+    {
+        const int nc = a.ncols_in;
+        const int total = nc * G;
+        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+            const int kx = idx / G, ll = idx - kx * G;
+            C p0, p1;
+            if (MODE == MODE_PAIR) {
+                p0 = a.P0[(size_t)kx * pitch + y0 + ll];
+                p1 = a.P0[(size_t)kx * pitch + y0 + G + ll];
+            } else {
+                p0 = a.P0[(size_t)kx * pitch + y0 + ll];
+                p1 = a.P1[(size_t)kx * pitch + y0 + ll];
+            }
+            C* bl = lds + ll * LE;
+            const bool edge = (kx == 0) || (2 * kx == N);
+            if (edge) {
+                p0.y = 0;
+                p1.y = 0;
+            }
+            bl[lpad(kx)] = C{p0.x - p1.y, p0.y + p1.x};
+            if (!edge) bl[lpad(N - kx)] = C{p0.x + p1.y, p1.x - p0.y};
+        }
+        // zero the band no input column covers: kx in [nc, N - nc]
+        const int gap = N - 2 * nc + 1;
+        if (gap > 0) {
+            const int tz = gap * G;
+            for (int idx = threadIdx.x; idx < tz; idx += blockDim.x) {
+                const int q = idx / G, ll = idx - q * G;
+                lds[ll * LE + lpad(nc + q)] = C{0, 0};
+            }
+        }
+    }
+    C* buf = lds + l * LE;
+    C v[E];
+    F::template run_from_lds<+1>(v, buf, t, a.twx);
+
+    // epilogue
+    R sA[E], sB[E];
+    const size_t bA = (size_t)nx * rowA, bB = (size_t)nx * rowB;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int x = t + T * m;
+        const R tA = v[m].x, tB = v[m].y;
+        if (EPI == EPI_STORE) {
+            a.out_r[bA + x] = tA;
+            a.out_r[bB + x] = tB;
+        } else if (EPI == EPI_STORE_W) {
+            a.w_out[bA + x] = C{tA, tB};
+        } else {
+            double nbA, nbB;
+            if (EPI == EPI_RK1) {
+                a.k1[bA + x] = tA;
+                a.k1[bB + x] = tB;
+                nbA = a.b[bA + x] + a.c1 * (double)tA;
+                nbB = a.b[bB + x] + a.c1 * (double)tB;
+            } else if (EPI == EPI_RK2) {
+                nbA = a.b[bA + x] + a.c1 * (double)tA;
+                nbB = a.b[bB + x] + a.c1 * (double)tB;
+            } else if (EPI == EPI_RK3) {
+                nbA = a.b[bA + x] + a.c1 * ((double)a.k1[bA + x] + 3.0 * (double)tA);
+                nbB = a.b[bB + x] + a.c1 * ((double)a.k1[bB + x] + 3.0 * (double)tB);
+                a.b[bA + x] = nbA;
+                a.b[bB + x] = nbB;
+            } else if (EPI == EPI_AB2) {
+                if (a.first) {
+                    nbA = a.b[bA + x] + a.c1 * (double)tA;
+                    nbB = a.b[bB + x] + a.c1 * (double)tB;
+                } else {
+                    nbA = a.b[bA + x] + a.c1 * (1.5 * (double)tA - 0.5 * (double)a.fold[bA + x]);
+                    nbB = a.b[bB + x] + a.c1 * (1.5 * (double)tB - 0.5 * (double)a.fold[bB + x]);
+                }
+                a.fold[bA + x] = tA;
+                a.fold[bB + x] = tB;
+                a.b[bA + x] = nbA;
+                a.b[bB + x] = nbB;
+            } else if (EPI == EPI_SET) {
+                nbA = (double)tA;
+                nbB = (double)tB;
+                a.b[bA + x] = nbA;
+                a.b[bB + x] = nbB;
+            } else {  // EPI_ADD
+                nbA = a.b[bA + x] + (double)tA;
+                nbB = a.b[bB + x] + (double)tB;
+                a.b[bA + x] = nbA;
+                a.b[bB + x] = nbB;
+            }
+            sA[m] = (R)nbA;
+            sB[m] = (R)nbB;
+        }
+    }
+    if (EPI == EPI_STORE || EPI == EPI_STORE_W) return;
+    if (!a.emit) return;  // uniform across the grid
+
+    // next-stage fluxes of both rows: x-FFT of (u*b) + i(v*b)
+    for (int half = 0; half < 2; ++half) {
+        const size_t bR = half == 0 ? bA : bB;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int x = t + T * m;
+            const C uv = a.w[bR + x];
+            const R s = half == 0 ? sA[m] : sB[m];
+            v[m] = C{uv.x * s, uv.y * s};
+        }
+        F::template run<-1>(v, buf, t, a.twx);
+        F::to_lds(v, buf, t);
+        const int r0 = half == 0 ? y0 : (MODE == MODE_PAIR ? y0 + G : y0);
+        store_split<R, N, E>(lds, G, a.ncols_out, a.Fx, a.Fy, pitch, r0, r0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversion between this library's spectrum (kx <= nx/2 planes,
+// full ky) and the reference's rfft2 array (nx, ny/2+1), x fastest.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_to_ref(const cpx<R>* __restrict__ P, double2* __restrict__ out, Grid g) {
+    const int nyc = g.ny / 2 + 1;
+    const long total = (long)g.nx * nyc;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(q % g.nx), j = (int)(q / g.nx);
+        cpx<R> z;
+        if (2 * i <= g.nx) {
+            z = P[(size_t)i * g.pitch + j];
+        } else {
+            z = cconj(P[(size_t)(g.nx - i) * g.pitch + ((g.ny - j) % g.ny)]);
+        }
+        out[q] = double2{(double)z.x, (double)z.y};
+    }
+}
+
+// Reference half-plane -> Hermitian full plane in this library's layout:
+// ky rows 0 and ny/2 take (S(i) + conj S(-i)) / 2, which is exactly what the
+// reference's c2r along y keeps of them.
+template <typename R>
+__global__ void k_from_ref(const double2* __restrict__ in, cpx<R>* __restrict__ P, Grid g) {
+    const int ncol = g.nx / 2 + 1;
+    const long total = (long)ncol * g.ny;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q / g.ny), r = (int)(q % g.ny);
+        const int cm = (g.nx - c) % g.nx;
+        double2 z;
+        if (r == 0 || 2 * r == g.ny) {
+            const double2 p = in[(size_t)g.nx * r + c], m = in[(size_t)g.nx * r + cm];
+            z = double2{0.5 * (p.x + m.x), 0.5 * (p.y - m.y)};
+        } else if (2 * r < g.ny) {
+            z = in[(size_t)g.nx * r + c];
+        } else {
+            const double2 p = in[(size_t)g.nx * (g.ny - r) + cm];
+            z = double2{p.x, -p.y};
+        }
+        P[(size_t)c * g.pitch + r] = cpx<R>{(R)z.x, (R)z.y};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions (FP64, two pass: per-block partials, summed on the host in a
+// fixed order so results are deterministic).
+// ---------------------------------------------------------------------------
+template <int W>
+__device__ __forceinline__ void block_sum(double (&v)[W], double* out) {
+    __shared__ double red[W][256 / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        double s = v[k];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        if (lane == 0) red[k][wv] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            double s = 0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[k][i];
+            out[(size_t)blockIdx.x * W + k] = s;
+        }
+    }
+}
+
+// sum b, sum b^2, sum non-finite count
+__global__ __launch_bounds__(256) void k_stats(const double* __restrict__ b, long n, double* partial) {
+    double v[3] = {0, 0, 0};
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const double x = b[q];
+        if (isfinite(x)) {
+            v[0] += x;
+            v[1] += x * x;
+        } else {
+            v[2] += 1.0;
+        }
+    }
+    block_sum<3>(v, partial);
+}
+
+// parsevalsum2 weights (src/transforms.jl:239-261) on this library's planes:
+// sum over ky in [0, ny/2] and kx in [0, nx/2] of w(kx) |P|^2, w = 1 at kx = 0
+// and nx/2, 2 otherwise.
+template <typename R>
+__global__ __launch_bounds__(256) void k_parseval_ref(const cpx<R>* __restrict__ P, Grid g, double* partial) {
+    double v[1] = {0};
+    const int ncol = g.nx / 2 + 1, nrow = g.ny / 2 + 1;
+    const long total = (long)ncol * nrow;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < total; q += (long)gridDim.x * 256) {
+        const int c = (int)(q / nrow), r = (int)(q % nrow);
+        const cpx<R> z = P[(size_t)c * g.pitch + r];
+        const double wgt = (c == 0 || 2 * c == g.nx) ? 1.0 : 2.0;
+        v[0] += wgt * ((double)z.x * z.x + (double)z.y * z.y);
+    }
+    block_sum<1>(v, partial);
+}
+
+// ---------------------------------------------------------------------------
+// small elementwise helpers
+// ---------------------------------------------------------------------------
+template <typename A, typename B>
+__global__ void k_convert(const A* __restrict__ in, B* __restrict__ out, long n) {
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        out[q] = (B)in[q];
+}
+
+template <typename R>
+__global__ void k_w_convert(const double2* __restrict__ in, cpx<R>* __restrict__ out, long n) {
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        out[q] = cpx<R>{(R)in[q].x, (R)in[q].y};
+}
+
+template <typename R>
+__global__ void k_w_split(const cpx<R>* __restrict__ w, double* __restrict__ u, double* __restrict__ v, long n) {
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        u[q] = (double)w[q].x;
+        v[q] = (double)w[q].y;
+    }
+}
+
+}  // namespace ssg

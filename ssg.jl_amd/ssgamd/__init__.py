@@ -1,0 +1,361 @@
+"""Python mirror of SSG.jl's time-step API over the HIP C ABI (include/ssg.h).
+
+The names follow the reference so that host code and tests read like its own:
+
+=========================================  ==========================================
+reference (SSG.jl)                          here
+=========================================  ==========================================
+``make_domain`` (src/domain.jl:217-230)     :func:`make_domain` -> :class:`Domain`
+``allocate_fields`` (src/fields.jl:72)      :func:`allocate_fields` -> :class:`Fields`
+``TimeParams`` (src/timestep.jl:19-40)      :class:`TimeParams`
+``TimeState`` (src/timestep.jl:45-86)       :class:`TimeState`
+``timestep!`` (src/timestep.jl:267-285)     :func:`timestep`
+``compute_tendency!`` (src/nonlinear.jl:22) :func:`compute_tendency`
+``compute_jacobian!`` (src/nonlinear.jl:42) :func:`compute_jacobian`
+``rfft_2d!``/``irfft_2d!`` (transforms:42)  :func:`rfft_2d` / :func:`irfft_2d`
+``apply_spectral_filter!`` (timestep:247)   :func:`apply_spectral_filter`
+=========================================  ==========================================
+
+All compute runs in ``libssg_hip.so``; there is no CPU fallback.  Importing
+works without a GPU (so the ABI can be inspected), but creating a domain
+needs the library and a HIP device and raises otherwise.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_PKG, "..", "lib", "libssg_hip.so"))
+
+SSG_OK = 0
+SSG_ERR_ARG = -1
+SSG_ERR_HIP = -2
+SSG_ERR_NONFINITE = -3
+SSG_ERR_UNSUPPORTED = -4
+SSG_ERR_NOT_CONVERGED = -5
+
+SSG_FP64 = 0
+SSG_FP32_MIXED = 1
+
+FIELD_B, FIELD_PHI, FIELD_U, FIELD_V = 0, 1, 2, 3
+
+# Every symbol include/ssg.h declares
+EXPORTED_SYMBOLS = (
+    "ssg_create", "ssg_destroy", "ssg_last_error", "ssg_sync", "ssg_set_field", "ssg_get_field",
+    "ssg_set_field_device", "ssg_get_field_device", "ssg_timestep", "ssg_tendency", "ssg_jacobian",
+    "ssg_rfft_2d", "ssg_irfft_2d", "ssg_apply_exponential_filter", "ssg_diagnostics", "ssg_last_timing",
+    "ssg_profile", "ssg_profile_read",
+)
+
+
+class TimeScheme(enum.IntEnum):
+    """``@enum TimeScheme`` (src/timestep.jl:10-14)."""
+    AB2_LowStorage = 0
+    RK3 = 1
+    RK3_LowStorage = 2
+
+
+AB2_LowStorage = TimeScheme.AB2_LowStorage
+RK3 = TimeScheme.RK3
+RK3_LowStorage = TimeScheme.RK3_LowStorage
+
+
+class SSGError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ssg error {code}: {msg}")
+        self.code = code
+
+
+class _DomainDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("lz", ctypes.c_double),
+                ("precision", ctypes.c_int32), ("device", ctypes.c_int32)]
+
+
+class _TimeParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("scheme", ctypes.c_int32), ("filter_freq", ctypes.c_int32),
+                ("filter_strength", ctypes.c_double), ("filter_order", ctypes.c_int32),
+                ("filter_cutoff", ctypes.c_double), ("ab2_reference_aliasing", ctypes.c_int32),
+                ("check_finite", ctypes.c_int32)]
+
+
+class _TimeState(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_double), ("step", ctypes.c_int64), ("dt_actual", ctypes.c_double)]
+
+
+class _Diag(ctypes.Structure):
+    _fields_ = [("sum_b", ctypes.c_double), ("sum_b2", ctypes.c_double), ("parseval_b", ctypes.c_double),
+                ("ke_surface", ctypes.c_double), ("nonfinite", ctypes.c_int64)]
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def load_library(path: str | None = None):
+    """Load libssg_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(p)
+    vp = ctypes.c_void_p
+    lib.ssg_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(_DomainDesc)]
+    lib.ssg_destroy.argtypes = [vp]
+    lib.ssg_last_error.restype = ctypes.c_char_p
+    lib.ssg_last_error.argtypes = [vp]
+    lib.ssg_sync.argtypes = [vp]
+    for n in ("ssg_set_field", "ssg_get_field", "ssg_set_field_device", "ssg_get_field_device"):
+        getattr(lib, n).argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int64]
+    lib.ssg_timestep.argtypes = [vp, ctypes.POINTER(_TimeParams), ctypes.POINTER(_TimeState), ctypes.c_int64]
+    lib.ssg_tendency.argtypes = [vp, _dp, ctypes.c_int32]
+    lib.ssg_jacobian.argtypes = [vp, _dp]
+    lib.ssg_rfft_2d.argtypes = [vp, _dp, _dp]
+    lib.ssg_irfft_2d.argtypes = [vp, _dp, _dp]
+    lib.ssg_apply_exponential_filter.argtypes = [vp, ctypes.c_double, ctypes.c_int32, ctypes.c_double]
+    lib.ssg_diagnostics.argtypes = [vp, ctypes.POINTER(_Diag)]
+    lib.ssg_last_timing.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int64)]
+    lib.ssg_profile.argtypes = [vp, ctypes.c_int32]
+    lib.ssg_profile_read.argtypes = [vp, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), _dp]
+    for n in EXPORTED_SYMBOLS:
+        getattr(lib, n).restype = ctypes.c_char_p if n == "ssg_last_error" else ctypes.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _f64(a, shape=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class Domain:
+    """Grid + device context (``Domain``, src/domain.jl:33-186).
+
+    Real fields are numpy arrays of shape ``(ny, nx)`` (x fastest, the
+    reference's column-major ``(nx, ny)``).  Spectra in the reference layout
+    are complex ``(ny//2+1, nx)`` arrays.
+    """
+
+    def __init__(self, nx: int, ny: int, nz: int = 1, Lx: float = 2 * np.pi, Ly: float = 2 * np.pi,
+                 Lz: float = 1.0, precision: int = SSG_FP64, device: int = 0):
+        self.lib = load_library()
+        self.Nx, self.Ny, self.Nz = int(nx), int(ny), int(nz)
+        self.Lx, self.Ly, self.Lz = float(Lx), float(Ly), float(Lz)
+        self.precision = int(precision)
+        desc = _DomainDesc(self.Nx, self.Ny, self.Nz, self.Lx, self.Ly, self.Lz, self.precision, int(device))
+        h = ctypes.c_void_p()
+        rc = self.lib.ssg_create(ctypes.byref(h), ctypes.byref(desc))
+        if rc != SSG_OK:
+            raise SSGError(rc, "ssg_create failed (see stderr)")
+        self._h = h
+        self.nyc = self.Ny // 2 + 1
+        # src/domain.jl:145-162
+        self.x = (self.Lx / self.Nx) * np.arange(self.Nx)
+        self.y = (self.Ly / self.Ny) * np.arange(self.Ny)
+        self.kx = 2 * np.pi * np.fft.fftfreq(self.Nx, self.Lx / self.Nx)
+        self.ky = 2 * np.pi * np.fft.rfftfreq(self.Ny, self.Ly / self.Ny)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def check(self, rc: int):
+        if rc != SSG_OK:
+            msg = self.lib.ssg_last_error(self._h)
+            raise SSGError(rc, msg.decode() if msg else "")
+        return rc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.ssg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- field transfer ----
+    def set_field(self, field: int, a):
+        a = _f64(a, (self.Ny, self.Nx))
+        self.check(self.lib.ssg_set_field(self._h, field, a.ctypes.data, a.size))
+
+    def get_field(self, field: int) -> np.ndarray:
+        out = np.empty((self.Ny, self.Nx))
+        self.check(self.lib.ssg_get_field(self._h, field, out.ctypes.data, out.size))
+        return out
+
+    def set_field_device(self, field: int, ptr: int, n: int):
+        self.check(self.lib.ssg_set_field_device(self._h, field, ctypes.c_void_p(ptr), n))
+
+    def get_field_device(self, field: int, ptr: int, n: int):
+        self.check(self.lib.ssg_get_field_device(self._h, field, ctypes.c_void_p(ptr), n))
+
+    def profile(self, enable: bool):
+        self.check(self.lib.ssg_profile(self._h, int(bool(enable))))
+
+    def profile_read(self, kind: int):
+        """(milliseconds, launches, algorithmic bytes) of one kernel class."""
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_int64()
+        self.check(self.lib.ssg_profile_read(self._h, int(kind), ctypes.byref(ms), ctypes.byref(n), ctypes.byref(by)))
+        return ms.value, n.value, by.value
+
+    def sync(self):
+        self.check(self.lib.ssg_sync(self._h))
+
+    def last_timing(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self.check(self.lib.ssg_last_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+
+def make_domain(Nx: int, Ny: int, Nz: int = 1, *, Lx=2 * np.pi, Ly=2 * np.pi, Lz=1.0,
+                precision: int = SSG_FP64, device: int = 0) -> Domain:
+    """``make_domain`` (src/domain.jl:217-230) on one GPU."""
+    return Domain(Nx, Ny, Nz, Lx, Ly, Lz, precision, device)
+
+
+class Fields:
+    """Device-resident ``Fields`` (src/fields.jl:28-60): ``bs`` and ``phis``
+    read and write through the ABI."""
+
+    def __init__(self, domain: Domain):
+        self.domain = domain
+
+    @property
+    def bs(self) -> np.ndarray:
+        return self.domain.get_field(FIELD_B)
+
+    @bs.setter
+    def bs(self, a):
+        self.domain.set_field(FIELD_B, a)
+
+    @property
+    def phis(self) -> np.ndarray:
+        return self.domain.get_field(FIELD_PHI)
+
+    @phis.setter
+    def phis(self, a):
+        self.domain.set_field(FIELD_PHI, a)
+
+    @property
+    def u(self) -> np.ndarray:
+        return self.domain.get_field(FIELD_U)
+
+    @property
+    def v(self) -> np.ndarray:
+        return self.domain.get_field(FIELD_V)
+
+
+def allocate_fields(domain: Domain) -> Fields:
+    """``allocate_fields`` (src/fields.jl:72-116)."""
+    return Fields(domain)
+
+
+@dataclass
+class TimeParams:
+    """``TimeParams`` (src/timestep.jl:19-40).  ``filter_order`` and
+    ``filter_cutoff`` default to the legacy path's fixed 4 and 0.65
+    (src/filter.jl:525-532)."""
+    dt: float
+    scheme: TimeScheme = AB2_LowStorage
+    filter_freq: int = 10
+    filter_strength: float = 0.1
+    cfl_safety: float = 0.5
+    max_dt: float = 0.1
+    min_dt: float = 1e-6
+    adaptive_dt: bool = False
+    filter_order: int = 4
+    filter_cutoff: float = 0.65
+    ab2_reference_aliasing: bool = False
+    check_finite: bool = False
+
+    def _c(self) -> _TimeParams:
+        if self.adaptive_dt:
+            raise SSGError(SSG_ERR_UNSUPPORTED, "adaptive_dt is not supported on the GPU path")
+        return _TimeParams(float(self.dt), int(self.scheme), int(self.filter_freq), float(self.filter_strength),
+                           int(self.filter_order), float(self.filter_cutoff), int(bool(self.ab2_reference_aliasing)),
+                           int(bool(self.check_finite)))
+
+
+class TimeState:
+    """``TimeState`` scalars (src/timestep.jl:45-86); the stage buffers and
+    the AB2 history live on the device."""
+
+    def __init__(self, initial_time: float = 0.0, fields: Fields | None = None):
+        self._c = _TimeState(float(initial_time), 0, 0.0)
+
+    @property
+    def t(self):
+        return self._c.t
+
+    @property
+    def step(self):
+        return self._c.step
+
+    @property
+    def dt_actual(self):
+        return self._c.dt_actual
+
+
+def timestep(fields: Fields, domain: Domain, params: TimeParams, state: TimeState, nsteps: int = 1) -> float:
+    """``timestep!`` (src/timestep.jl:267-285), ``nsteps`` times.  Returns dt."""
+    p = params._c()
+    domain.check(domain.lib.ssg_timestep(domain.handle, ctypes.byref(p), ctypes.byref(state._c), int(nsteps)))
+    return state.dt_actual
+
+
+def compute_tendency(fields: Fields, domain: Domain, ab2_reference_aliasing: bool = False) -> np.ndarray:
+    """``compute_tendency!`` (src/nonlinear.jl:22-36) of the current fields."""
+    out = np.empty((domain.Ny, domain.Nx))
+    domain.check(domain.lib.ssg_tendency(domain.handle, out.ctypes.data_as(_dp), int(ab2_reference_aliasing)))
+    return out
+
+
+def compute_jacobian(fields: Fields, domain: Domain) -> np.ndarray:
+    """``compute_jacobian!`` (src/nonlinear.jl:42-57): -J(phi, b)."""
+    out = np.empty((domain.Ny, domain.Nx))
+    domain.check(domain.lib.ssg_jacobian(domain.handle, out.ctypes.data_as(_dp)))
+    return out
+
+
+def rfft_2d(domain: Domain, a) -> np.ndarray:
+    """``rfft_2d!`` (src/transforms.jl:42-45): reference layout (ny//2+1, nx)."""
+    a = _f64(a, (domain.Ny, domain.Nx))
+    out = np.empty((domain.nyc, domain.Nx), dtype=np.complex128)
+    domain.check(domain.lib.ssg_rfft_2d(domain.handle, a.ctypes.data_as(_dp), out.ctypes.data_as(_dp)))
+    return out
+
+
+def irfft_2d(domain: Domain, s) -> np.ndarray:
+    """``irfft_2d!`` (src/transforms.jl:53-56)."""
+    s = np.ascontiguousarray(s, dtype=np.complex128).reshape(domain.nyc, domain.Nx)
+    out = np.empty((domain.Ny, domain.Nx))
+    domain.check(domain.lib.ssg_irfft_2d(domain.handle, s.ctypes.data_as(_dp), out.ctypes.data_as(_dp)))
+    return out
+
+
+def apply_spectral_filter(fields: Fields, domain: Domain, strength: float, order: int = 4,
+                          cutoff: float = 0.65) -> None:
+    """``apply_spectral_filter!`` (src/timestep.jl:247-260) on ``bs``."""
+    domain.check(domain.lib.ssg_apply_exponential_filter(domain.handle, float(strength), int(order), float(cutoff)))
+
+
+def diagnostics(domain: Domain) -> dict:
+    d = _Diag()
+    domain.check(domain.lib.ssg_diagnostics(domain.handle, ctypes.byref(d)))
+    return {"sum_b": d.sum_b, "sum_b2": d.sum_b2, "parseval_b": d.parseval_b, "ke_surface": d.ke_surface,
+            "nonfinite": d.nonfinite}

@@ -1,0 +1,215 @@
+"""GPU parity: the HIP library (through its C ABI) against the FP64 oracle on
+identical seeded inputs.  FP64 bar: relative max error 1e-11 (FFT roundoff
+of a different factorisation; bitwise equality is not defined across FFT
+algorithms).  FP32-mixed bar: relative error 1e-5 (the precision metric's
+stated tolerance)."""
+import numpy as np
+import pytest
+
+from oracle import Oracle
+from oracle.witness import Witness
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-11
+TOL32 = 1e-5
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def grid(nx, ny, lx=2 * np.pi, ly=2 * np.pi):
+    x = lx / nx * np.arange(nx)
+    y = ly / ny * np.arange(ny)
+    return np.meshgrid(x, y)
+
+
+SIZES = [(8, 8), (16, 16), (32, 16), (16, 64), (64, 64), (128, 128), (256, 128)]
+
+
+@pytest.mark.parametrize("nx,ny", SIZES)
+def test_rfft2_irfft2(gpu, nx, ny):
+    print(f"rfft2 {nx}x{ny}", flush=True)
+    rng = np.random.default_rng(nx + 3 * ny)
+    d = gpu.make_domain(nx, ny, 1)
+    o = Oracle(nx, ny)
+    a = rng.standard_normal((ny, nx))
+    s = gpu.rfft_2d(d, a)
+    assert rel(s, o.rfft2(a)) < TOL64
+    # irfft of an arbitrary (non-Hermitian) reference-layout array: the
+    # reference's c2r convention must be reproduced
+    z = rng.standard_normal(s.shape) + 1j * rng.standard_normal(s.shape)
+    assert rel(gpu.irfft_2d(d, z), o.irfft2(z)) < TOL64
+    assert rel(gpu.irfft_2d(d, s), a) < TOL64
+
+
+@pytest.mark.parametrize("nx,ny", SIZES)
+@pytest.mark.parametrize("prec", [0, 1])
+def test_tendency_and_jacobian(gpu, nx, ny, prec):
+    print(f"tendency {nx}x{ny} prec={prec}", flush=True)
+    rng = np.random.default_rng(11 * nx + ny)
+    d = gpu.make_domain(nx, ny, 1, precision=prec)
+    f = gpu.allocate_fields(d)
+    o = Oracle(nx, ny)
+    # deliberately not band-limited: exercises the Nyquist rows and columns
+    psi = rng.standard_normal((ny, nx))
+    b = rng.standard_normal((ny, nx))
+    f.phis = psi
+    f.bs = b
+    tol = TOL64 if prec == 0 else TOL32 * 10
+    w = Witness(nx, ny)
+    assert rel(f.u, -w.irfft2(w.ddy(w.rfft2(psi)))) < (TOL64 if True else tol)
+    assert rel(f.v, w.irfft2(w.ddx(w.rfft2(psi)))) < TOL64
+    assert rel(gpu.compute_tendency(f, d), o.tendency(psi, b)) < tol
+    assert rel(gpu.compute_tendency(f, d, True), o.tendency(psi, b, True)) < tol
+    assert rel(gpu.compute_jacobian(f, d), o.jacobian(psi, b)) < tol
+
+
+def test_k1_k2_closed_forms(gpu):
+    X, Y = grid(32, 32)
+    d = gpu.make_domain(32, 32, 4)
+    f = gpu.allocate_fields(d)
+    f.phis = np.sin(X) * np.cos(Y)
+    f.bs = np.cos(X) * np.sin(Y)
+    expect = np.sin(X) ** 2 * np.sin(Y) ** 2 - np.cos(X) ** 2 * np.cos(Y) ** 2
+    assert np.abs(gpu.compute_jacobian(f, d) - expect).max() < 1e-13
+    X, Y = grid(16, 16)
+    d = gpu.make_domain(16, 16, 4)
+    f = gpu.allocate_fields(d)
+    f.phis = 0.1 * np.sin(X) * np.cos(Y)
+    f.bs = 0.1 * np.cos(X) * np.sin(Y)
+    expect = -0.01 * (np.cos(X) ** 2 * np.cos(Y) ** 2 - np.sin(X) ** 2 * np.sin(Y) ** 2)
+    assert np.abs(gpu.compute_tendency(f, d) - expect).max() < 1e-15
+
+
+@pytest.mark.parametrize("nx,ny", [(32, 32), (64, 32), (128, 128)])
+def test_filter(gpu, nx, ny):
+    rng = np.random.default_rng(5)
+    d = gpu.make_domain(nx, ny, 1)
+    f = gpu.allocate_fields(d)
+    o = Oracle(nx, ny)
+    b = rng.standard_normal((ny, nx))
+    f.bs = b
+    gpu.apply_spectral_filter(f, d, 36.0, 8, 0.65)
+    assert rel(f.bs, o.filter(b, 36.0, 8, 0.65)) < TOL64
+    f.bs = b
+    gpu.apply_spectral_filter(f, d, 1.0)
+    assert rel(f.bs, o.filter(b, 1.0)) < TOL64
+
+
+def run_pair(gpu, nx, ny, scheme, nsteps, dt, prec=0, filter_freq=10, strength=36.0, order=8, cutoff=0.65,
+             aliasing=False, seed=0, chunks=(None,)):
+    from ssgamd.ics import spectral_b0, streamfunction_from_b
+    b0 = spectral_b0(nx, ny, seed=seed)
+    psi = streamfunction_from_b(b0)
+    d = gpu.make_domain(nx, ny, 1, precision=prec)
+    f = gpu.allocate_fields(d)
+    f.phis = psi
+    f.bs = b0
+    params = gpu.TimeParams(dt, scheme=scheme, filter_freq=filter_freq, filter_strength=strength,
+                            filter_order=order, filter_cutoff=cutoff, ab2_reference_aliasing=aliasing)
+    st = gpu.TimeState(0.0, f)
+    done = 0
+    for ch in chunks:
+        n = nsteps - done if ch is None else ch
+        gpu.timestep(f, d, params, st, n)
+        done += n
+    assert done == nsteps and st.step == nsteps
+    o = Oracle(nx, ny)
+    bo = b0.copy()
+    for _ in range(nsteps):
+        assert o.timestep(bo, psi, int(scheme), dt, filter_freq, strength, order, cutoff, int(aliasing)) == 0
+    assert abs(st.t - o.t.value) < 1e-12
+    return f.bs, bo
+
+
+@pytest.mark.parametrize("nx,ny", [(32, 32), (64, 128), (128, 128)])
+def test_rk3_steps_match_oracle(gpu, nx, ny):
+    print(f"rk3 {nx}x{ny}", flush=True)
+    g, o = run_pair(gpu, nx, ny, gpu.RK3, 25, 1e-3, chunks=(7, 1, None))
+    assert rel(g, o) < TOL64
+
+
+@pytest.mark.parametrize("aliasing", [False, True])
+def test_ab2_steps_match_oracle(gpu, aliasing):
+    g, o = run_pair(gpu, 64, 64, gpu.AB2_LowStorage, 23, 2.5e-4, aliasing=aliasing, chunks=(1, 5, None))
+    assert rel(g, o) < TOL64
+
+
+def test_legacy_filter_cadence(gpu):
+    """timestep! with the reference defaults: filter every 10 steps, α = 0.1, p = 4, c = 0.65."""
+    g, o = run_pair(gpu, 64, 64, gpu.RK3, 20, 1e-3, strength=0.1, order=4, cutoff=0.65)
+    assert rel(g, o) < TOL64
+
+
+def test_fp32_mixed_rk3_against_fp64(gpu):
+    g, o = run_pair(gpu, 128, 128, gpu.RK3, 50, 1e-3, prec=1)
+    err = np.linalg.norm(g - o) / np.linalg.norm(o)
+    print("fp32-mixed rel-L2 after 50 steps", err)
+    assert err < TOL32
+
+
+def test_diagnostics(gpu):
+    rng = np.random.default_rng(9)
+    nx, ny = 64, 32
+    d = gpu.make_domain(nx, ny, 4)
+    f = gpu.allocate_fields(d)
+    o = Oracle(nx, ny, 4)
+    b = rng.standard_normal((ny, nx))
+    psi = rng.standard_normal((ny, nx))
+    f.bs = b
+    f.phis = psi
+    dg = gpu.diagnostics(d)
+    assert abs(dg["sum_b"] - b.sum()) < 1e-10
+    assert abs(dg["sum_b2"] - (b * b).sum()) < 1e-9
+    assert abs(dg["parseval_b"] - o.parsevalsum2(o.rfft2(b))) < 1e-10 * abs(dg["parseval_b"])
+    u, v = f.u, f.v
+    ke = 0.5 * o.parsevalsum2(o.rfft2(u)) + 0.5 * o.parsevalsum2(o.rfft2(v))
+    assert abs(dg["ke_surface"] - ke) < 1e-10 * ke
+    assert dg["nonfinite"] == 0
+
+
+def test_error_paths(gpu):
+    d = gpu.make_domain(16, 16, 1)
+    f = gpu.allocate_fields(d)
+    st = gpu.TimeState(0.0, f)
+    with pytest.raises(gpu.SSGError) as e:
+        gpu.timestep(f, d, gpu.TimeParams(0.1, scheme=gpu.RK3_LowStorage), st)
+    assert e.value.code == gpu.SSG_ERR_UNSUPPORTED
+    with pytest.raises(gpu.SSGError) as e:
+        gpu.timestep(f, d, gpu.TimeParams(0.1, scheme=gpu.RK3, filter_strength=-1.0), st)
+    assert e.value.code == gpu.SSG_ERR_ARG
+    b = np.zeros((16, 16))
+    b[3, 4] = np.nan
+    f.bs = b
+    with pytest.raises(gpu.SSGError) as e:
+        gpu.timestep(f, d, gpu.TimeParams(0.01, scheme=gpu.RK3, check_finite=True), st)
+    assert e.value.code == gpu.SSG_ERR_NONFINITE
+    with pytest.raises(gpu.SSGError):
+        gpu.make_domain(12, 16, 1)
+    with pytest.raises(gpu.SSGError):
+        d.set_field(gpu.FIELD_B, np.zeros(17))
+
+
+def test_full_size_properties(gpu):
+    """1024² (the precision config's grid): Σb is conserved to roundoff and a
+    rerun is bitwise identical."""
+    from ssgamd.ics import spectral_b0, streamfunction_from_b
+    nx = ny = 1024
+    b0 = spectral_b0(nx, ny, seed=0)
+    psi = streamfunction_from_b(b0)
+    outs = []
+    for prec in (0, 0, 1):
+        d = gpu.make_domain(nx, ny, 32, precision=prec)
+        f = gpu.allocate_fields(d)
+        f.phis = psi
+        f.bs = b0
+        st = gpu.TimeState(0.0, f)
+        gpu.timestep(f, d, gpu.TimeParams(2.5e-4, scheme=gpu.RK3, filter_freq=10, filter_strength=36.0,
+                                          filter_order=8, filter_cutoff=0.65), st, 20)
+        outs.append(f.bs)
+        d.close()
+    assert np.array_equal(outs[0], outs[1])
+    assert abs(outs[0].sum() - b0.sum()) < 1e-9
+    assert np.linalg.norm(outs[2] - outs[0]) / np.linalg.norm(outs[0]) < TOL32
