@@ -50,6 +50,7 @@ def run_gpu(ssg, prec, b0, psi, nx, ny, nz, params, warmup, steps, profile):
     d.sync()
     wall = time.perf_counter() - t0
     gpu_ms, launches = d.last_timing()
+    tune = d.tuning()
     b_end = f.bs
     prof = None
     if profile:
@@ -58,7 +59,7 @@ def run_gpu(ssg, prec, b0, psi, nx, ny, nz, params, warmup, steps, profile):
         prof = {k: d.profile_read(k) for k in KIND_NAMES}
         d.profile(False)
     d.close()
-    return wall, gpu_ms, launches, b_end, prof
+    return wall, gpu_ms, launches, b_end, prof, tune
 
 
 def load_traffic(workload):
@@ -129,7 +130,7 @@ def main():
         log(f"bench: {name} {args.steps / res[name][0]:.1f} steps/s (wall), gpu {res[name][1]:.2f} ms")
 
     main_name = "fp32" if "fp32" in res else "fp64"
-    wall, gpu_ms, launches, b_end, prof = res[main_name]
+    wall, gpu_ms, launches, b_end, prof, tune = res[main_name]
     value = args.steps / wall
     out = {
         "metric": METRIC,
@@ -171,6 +172,7 @@ def main():
         out["kernels"] = {k: {"avg_us": 1e3 * v["ms"] / max(v["launches"], 1), "launches": v["launches"],
                               "GBps": v["bytes"] / max(v["ms"], 1e-12) / 1e6} for k, v in kinds.items()}
         out["step_algorithmic_GBps"] = tot_b / (tot_ms / 1e3) / 1e9 if tot_ms > 0 else None
+    out["autotune"] = {k: v[5] for k, v in res.items()}
     if not args.no_cpu:
         log("bench: CPU baseline (oracle)")
         dt = args.dt

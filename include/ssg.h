@@ -145,6 +145,13 @@ int ssg_last_timing(ssg_ctx* ctx, double* ms, int64_t* launches);
    forward (k_rows_fwd), 1 = column pass (k_cols), 2 = row pass inverse with
    fused stage epilogue (k_rows_inv). */
 int ssg_profile(ssg_ctx* ctx, int32_t enable);
+
+/* Autotuned launch shape of the three line-FFT kernel classes for one path
+   (0 = FP64, 1 = FP32-mixed): ept[3] receives the elements per thread in
+   use; us[9] (if not NULL) the measured microseconds per launch of the
+   candidates 4, 8, 16 per class (-1 where not timed).  Tuning runs on the
+   first ssg_timestep of a path. */
+int ssg_tuning(ssg_ctx* ctx, int32_t path, int32_t* ept, double* us);
 int ssg_profile_read(ssg_ctx* ctx, int32_t kind, double* total_ms, int64_t* count, double* bytes);
 
 #ifdef __cplusplus

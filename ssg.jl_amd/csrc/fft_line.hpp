@@ -94,19 +94,56 @@ __host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int lds_line_elems(int n) { return n + n / 16 + 1; }
 
+// number of twiddle factors a thread applies in the stages after the first,
+// and the offset of stage NS within that list
+__host__ __device__ constexpr int tw_count_from(int n, int e, int ns) {
+    return ns >= n ? 0 : (cmin(e, n / ns) - 1) * (e / cmin(e, n / ns)) + tw_count_from(n, e, ns * cmin(e, n / ns));
+}
+__host__ __device__ constexpr int tw_offset(int n, int e, int ns) {
+    return tw_count_from(n, e, e) - tw_count_from(n, e, ns);
+}
+
 // Stockham line FFT.  v[] holds x[t + T*m] on entry and X[t + T*m] on exit.
-// `buf` is this line's LDS buffer (lds_line_elems(N) entries).  tw[m] =
-// exp(-2*pi*i*m/N), m < N; the inverse uses its conjugate.  Every thread of
-// the block must call this (it contains block barriers).
+// `buf` is this line's LDS buffer (lds_line_elems(N) entries).  The twiddle
+// factors a thread needs depend only on t, so they are fetched once per
+// kernel into registers (Twiddles::load, from tw[m] = exp(-2*pi*i*m/N)) and
+// shared by every forward and inverse transform of the kernel; loading them
+// up front keeps global-memory latency out of the barrier-separated stages.
+// Every thread of the block must call run() (it contains block barriers).
 template <typename R, int N, int E> struct LineFFT {
     static constexpr int T = N / E;
+    static constexpr int NTW = tw_count_from(N, E, E);
     using C = cpx<R>;
 
+    struct Twiddles {
+        C w[NTW > 0 ? NTW : 1];
+
+        template <int NS>
+        __device__ __forceinline__ void load_stage(int t, const C* __restrict__ tab) {
+            if constexpr (NS < N) {
+                constexpr int RAD = cmin(E, N / NS);
+                constexpr int Q = E / RAD;
+                constexpr int OFF = tw_offset(N, E, NS);
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    const int k = (t + T * q) % NS;
+#pragma unroll
+                    for (int r = 1; r < RAD; ++r) w[OFF + q * (RAD - 1) + r - 1] = tab[(r * k) * (N / (NS * RAD))];
+                }
+                load_stage<NS * RAD>(t, tab);
+            }
+        }
+        __device__ __forceinline__ void load(int t, const C* __restrict__ tab) {
+            if constexpr (T > 1) load_stage<E>(t, tab);
+        }
+    };
+
     template <int DIR, int NS>
-    __device__ __forceinline__ static void stage(C* v, C* buf, int t, const C* __restrict__ tw) {
+    __device__ __forceinline__ static void stage(C* v, C* buf, int t, const Twiddles& tw) {
         constexpr int RAD = cmin(E, N / NS);
         constexpr int Q = E / RAD;
         constexpr bool LAST = (NS * RAD == N);
+        constexpr int OFF = tw_offset(N, E, NS);
         C u[E];
         __syncthreads();
 #pragma unroll
@@ -123,7 +160,7 @@ template <typename R, int N, int E> struct LineFFT {
             C* w = u + q * RAD;
 #pragma unroll
             for (int r = 1; r < RAD; ++r) {
-                C f = tw[(r * k) * (N / (NS * RAD))];
+                C f = tw.w[OFF + q * (RAD - 1) + r - 1];
                 if (DIR > 0) f.y = -f.y;
                 w[r] = cmul(w[r], f);
             }
@@ -142,7 +179,7 @@ template <typename R, int N, int E> struct LineFFT {
 
     // Transform from registers.
     template <int DIR>
-    __device__ __forceinline__ static void run(C* v, C* buf, int t, const C* __restrict__ tw) {
+    __device__ __forceinline__ static void run(C* v, C* buf, int t, const Twiddles& tw) {
         SmallDFT<R, E, DIR>::run(v);
         if constexpr (T > 1) {
             __syncthreads();  // earlier readers of buf are done
@@ -154,7 +191,7 @@ template <typename R, int N, int E> struct LineFFT {
 
     // Transform whose input sits in LDS in natural order (buf[lpad(i)] = x[i]).
     template <int DIR>
-    __device__ __forceinline__ static void run_from_lds(C* v, C* buf, int t, const C* __restrict__ tw) {
+    __device__ __forceinline__ static void run_from_lds(C* v, C* buf, int t, const Twiddles& tw) {
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = buf[lpad(t + T * m)];

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <type_traits>
@@ -53,6 +54,11 @@ struct ssg_ctx {
     PathBuffers<double> d;
     PathBuffers<float> f;
     bool ab2_have_history = false;
+    // elements per thread of the line FFTs, per kernel class (0 rows_fwd,
+    // 1 cols, 2 rows_inv) and per path (0 FP64, 1 FP32); chosen by autotune()
+    int ept[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    bool tuned[2] = {false, false};
+    double tune_us[2][3][3] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int64_t last_launches = 0;
@@ -107,13 +113,44 @@ int with_size(int n, Fn&& fn) {
     }
 }
 
-constexpr int elems_per_thread(int n) { return n < 16 ? n : 16; }
+// Elements per thread E of a line FFT of length n: candidates 4, 8, 16 with
+// E <= n and n/E <= 256 threads per line.  Small lines use min(16, n) only.
+bool ept_valid(int n, int e) {
+    if (n < 64) return e == (n < 16 ? n : 16);
+    return (e == 4 || e == 8 || e == 16) && n / e <= 256;
+}
+int ept_default(int n) {
+    if (n < 64) return n < 16 ? n : 16;
+    return n / 8 <= 256 ? 8 : 16;
+}
+
+// run fn(integral_constant<N>, integral_constant<E>)
+template <typename Fn>
+int with_size_ept(int n, int e, Fn&& fn) {
+    if (!ept_valid(n, e)) return SSG_ERR_UNSUPPORTED;
+    return with_size(n, [&](auto ntag) -> int {
+        constexpr int N = decltype(ntag)::value;
+        if constexpr (N < 64) {
+            return fn(ntag, std::integral_constant<int, (N < 16 ? N : 16)>{});
+        } else {
+            if (e == 4) {
+                if constexpr (N / 4 <= 256) return fn(ntag, std::integral_constant<int, 4>{});
+                return SSG_ERR_UNSUPPORTED;
+            }
+            if (e == 8) {
+                if constexpr (N / 8 <= 256) return fn(ntag, std::integral_constant<int, 8>{});
+                return SSG_ERR_UNSUPPORTED;
+            }
+            return fn(ntag, std::integral_constant<int, 16>{});
+        }
+    });
+}
 
 // lines per block: at most 256 threads and kMaxLdsBytes of LDS, a power of
 // two that divides `rows`
 template <typename R>
-int lines_per_block(int n, int rows) {
-    const int T = n / elems_per_thread(n);
+int lines_per_block(int n, int e, int rows) {
+    const int T = n / e;
     const int le = lds_line_elems(n) * (int)sizeof(cpx<R>);
     int G = 1;
     while (2 * G * T <= 256 && 2 * G * le <= kMaxLdsBytes && rows % (2 * G) == 0) G *= 2;
@@ -130,18 +167,26 @@ PathBuffers<double>& bufs<double>(ssg_ctx* c) { return c->d; }
 template <>
 PathBuffers<float>& bufs<float>(ssg_ctx* c) { return c->f; }
 
+template <typename R>
+int ept_of(ssg_ctx* c, int kind, int n) {
+    const int p = std::is_same<R, float>::value ? 1 : 0;
+    const int e = c->ept[p][kind];
+    return (e && ept_valid(n, e)) ? e : ept_default(n);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <typename R, typename S, int SRC>
 int launch_rows_fwd(ssg_ctx* c, const S* f, const cpx<R>* w, cpx<R>* PA, cpx<R>* PB, int ncols) {
     PathBuffers<R>& B = bufs<R>(c);
-    return with_size(c->nx, [&](auto tag) -> int {
+    const int ept = ept_of<R>(c, 0, c->nx);
+    return with_size_ept(c->nx, ept, [&](auto tag, auto etag) -> int {
         constexpr int N = decltype(tag)::value;
-        constexpr int E = elems_per_thread(N);
+        constexpr int E = decltype(etag)::value;
         constexpr int T = N / E;
         const int rows_per_line = SRC == SRC_FLUX ? 1 : 2;
-        const int G = lines_per_block<R>(N, c->ny / rows_per_line);
+        const int G = lines_per_block<R>(N, E, c->ny / rows_per_line);
         const int nblk = c->ny / (rows_per_line * G);
         RowsFwdArgs<R, S> a{f, w, PA, PB, ncols, G, nblk, B.twx, c->g};
         const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<R>);
@@ -161,9 +206,10 @@ template <typename R, int OP>
 int launch_cols(ssg_ctx* c, const cpx<R>* inA, const cpx<R>* inB, cpx<R>* outA, cpx<R>* outB, int ncols,
                 R scale, R fstrength = 0, int forder = 1, R fcutoff = 1, int fdelta = 0) {
     PathBuffers<R>& B = bufs<R>(c);
-    return with_size(c->ny, [&](auto tag) -> int {
+    const int ept = ept_of<R>(c, 1, c->ny);
+    return with_size_ept(c->ny, ept, [&](auto tag, auto etag) -> int {
         constexpr int M = decltype(tag)::value;
-        constexpr int E = elems_per_thread(M);
+        constexpr int E = decltype(etag)::value;
         constexpr int T = M / E;
         const int le = lds_line_elems(M) * (int)sizeof(cpx<R>);
         int G = 1;
@@ -188,12 +234,13 @@ int launch_cols(ssg_ctx* c, const cpx<R>* inA, const cpx<R>* inB, cpx<R>* outA, 
 template <typename R, int MODE, int EPI>
 int launch_rows_inv(ssg_ctx* c, RowsInvArgs<R> a) {
     PathBuffers<R>& B = bufs<R>(c);
-    return with_size(c->nx, [&](auto tag) -> int {
+    const int ept = ept_of<R>(c, 2, c->nx);
+    return with_size_ept(c->nx, ept, [&](auto tag, auto etag) -> int {
         constexpr int N = decltype(tag)::value;
-        constexpr int E = elems_per_thread(N);
+        constexpr int E = decltype(etag)::value;
         constexpr int T = N / E;
         const int rows_per_line = MODE == MODE_PAIR ? 2 : 1;
-        const int G = lines_per_block<R>(N, c->ny / rows_per_line);
+        const int G = lines_per_block<R>(N, E, c->ny / rows_per_line);
         const int nblk = c->ny / (rows_per_line * G);
         a.G = G;
         a.nblk = nblk;
@@ -312,6 +359,13 @@ int tendency_cols(ssg_ctx* c, int quirk) {
     return launch_cols<R, OP_TEND>(c, B.Fx, B.Fy, B.Tp, nullptr, c->ncols_tend, scale);
 }
 
+template <typename R>
+int run_kind(ssg_ctx* c, int kind) {
+    if (kind == 0) return flux_forward<R>(c, c->ncols_tend);
+    if (kind == 1) return tendency_cols<R>(c, 0);
+    return stage_inverse<R, EPI_RK2>(c, 0.0, 1, 0);
+}
+
 // apply_spectral_filter! on b; FP64 sets b = irfft(σ b̂); FP32-mixed adds
 // the FP32 defect irfft((σ-1) b̂) to the FP64 b.  emit: refresh fluxes.
 template <typename R>
@@ -334,6 +388,69 @@ int filter_b(ssg_ctx* c, double strength, int order, double cutoff, int emit) {
     a.ncols_out = c->ncols_tend;
     if (delta) return launch_rows_inv<R, MODE_PAIR, EPI_ADD>(c, a);
     return launch_rows_inv<R, MODE_PAIR, EPI_SET>(c, a);
+}
+
+// Autotuner: for each kernel class time every valid elements-per-thread
+// candidate on this context's buffers (3 warm-up + 8 timed launches) and keep
+// the fastest.  The probed launches only write scratch planes, never the
+// prognostic state.  SSG_EPT="e_fwd,e_cols,e_inv" forces a choice and
+// SSG_TUNE=0 keeps the defaults.
+template <typename R>
+int run_kind(ssg_ctx* c, int kind);
+
+template <typename R>
+int autotune(ssg_ctx* c) {
+    const int p = std::is_same<R, float>::value ? 1 : 0;
+    if (c->tuned[p]) return SSG_OK;
+    c->tuned[p] = true;
+    if (const char* env = getenv("SSG_EPT")) {
+        int e0 = 0, e1 = 0, e2 = 0;
+        if (sscanf(env, "%d,%d,%d", &e0, &e1, &e2) == 3) {
+            c->ept[p][0] = e0;
+            c->ept[p][1] = e1;
+            c->ept[p][2] = e2;
+        }
+        return SSG_OK;
+    }
+    const char* tv = getenv("SSG_TUNE");
+    if ((tv && tv[0] == '0') || (long)c->nx * c->ny < 256L * 256L) return SSG_OK;
+    const bool prof_saved = c->prof;
+    const int64_t launches_saved = c->launches;
+    c->prof = false;
+    const int cands[3] = {4, 8, 16};
+    int rc = SSG_OK;
+    for (int kind = 0; kind < 3 && rc == SSG_OK; ++kind) {
+        const int n = kind == 1 ? c->ny : c->nx;
+        int best = 0;
+        double best_us = 1e30;
+        for (int ci = 0; ci < 3; ++ci) {
+            const int e = cands[ci];
+            c->tune_us[p][kind][ci] = -1.0;
+            if (!ept_valid(n, e)) continue;
+            c->ept[p][kind] = e;
+            for (int i = 0; i < 3 && rc == SSG_OK; ++i) rc = run_kind<R>(c, kind);
+            if (rc) break;
+            if (hipEventRecord(c->ev0, c->stream) != hipSuccess) { rc = SSG_ERR_HIP; break; }
+            for (int i = 0; i < 8 && rc == SSG_OK; ++i) rc = run_kind<R>(c, kind);
+            if (rc) break;
+            float ms = 0;
+            if (hipEventRecord(c->ev1, c->stream) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess ||
+                hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
+                rc = set_err(c, SSG_ERR_HIP, "autotune timing failed");
+                break;
+            }
+            const double us = 1e3 * ms / 8.0;
+            c->tune_us[p][kind][ci] = us;
+            if (us < best_us) {
+                best_us = us;
+                best = e;
+            }
+        }
+        c->ept[p][kind] = best;
+    }
+    c->prof = prof_saved;
+    c->launches = launches_saved;
+    return rc;
 }
 
 template <typename R>
@@ -648,6 +765,10 @@ int ssg_timestep(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, int64_
         return set_err(c, SSG_ERR_ARG, "AB2 step %lld needs the previous tendency; none is held", (long long)s->step);
     HIP_OK(c, hipSetDevice(c->device));
     if (nsteps == 0) return SSG_OK;
+    {
+        int trc = c->precision == SSG_FP32_MIXED ? autotune<float>(c) : autotune<double>(c);
+        if (trc) return trc;
+    }
     const int64_t l0 = c->launches;
     HIP_OK(c, hipEventRecord(c->ev0, c->stream));
     int rc = c->precision == SSG_FP32_MIXED ? run_steps<float>(c, p, s, nsteps) : run_steps<double>(c, p, s, nsteps);
@@ -691,6 +812,17 @@ int ssg_profile_read(ssg_ctx* c, int32_t kind, double* total_ms, int64_t* count,
     if (total_ms) *total_ms = ms;
     if (count) *count = n;
     if (bytes) *bytes = by;
+    return SSG_OK;
+}
+
+int ssg_tuning(ssg_ctx* c, int32_t path, int32_t* ept, double* us) {
+    if (!c || path < 0 || path > 1) return SSG_ERR_ARG;
+    for (int k = 0; k < 3; ++k) {
+        const int n = k == 1 ? c->ny : c->nx;
+        if (ept) ept[k] = (c->ept[path][k] && ept_valid(n, c->ept[path][k])) ? c->ept[path][k] : ept_default(n);
+        for (int i = 0; i < 3; ++i)
+            if (us) us[3 * k + i] = c->tuned[path] ? c->tune_us[path][k][i] : -1.0;
+    }
     return SSG_OK;
 }
 

@@ -160,6 +160,8 @@ __global__ __launch_bounds__(256) void k_rows_fwd(RowsFwdArgs<R, S> a) {
     const int G = a.G;
     const int blk = xcd_block(blockIdx.x, a.nblk);
     const int nx = a.g.nx;
+    typename F::Twiddles tw;
+    tw.load(t, a.twx);
     C v[E];
     int rowA0, rowB0;
     if (SRC == SRC_FLUX) {
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(256) void k_rows_fwd(RowsFwdArgs<R, S> a) {
         }
     }
     C* buf = lds + l * LE;
-    F::template run<-1>(v, buf, t, a.twx);
+    F::template run<-1>(v, buf, t, tw);
     F::to_lds(v, buf, t);
     store_split<R, N, E>(lds, G, a.ncols, a.PA, SRC == SRC_FLUX ? a.PB : a.PA, a.g.pitch, rowA0, rowB0);
 }
@@ -221,13 +223,23 @@ __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
     const bool live = c < a.ncols;
     const int pitch = a.g.pitch;
     C* buf = lds + l * LE;
+    typename F::Twiddles tw;
+    tw.load(t, a.twy);
+    constexpr bool TWO_IN = (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC);
     C va[E];
+    C vb[E];
     const C* srcA = a.inA + (size_t)(live ? c : 0) * pitch;
 #pragma unroll
     for (int m = 0; m < E; ++m) va[m] = live ? srcA[t + T * m] : C{0, 0};
+    if constexpr (TWO_IN) {
+        // issue the second plane's loads before the first transform
+        const C* srcB = a.inB + (size_t)(live ? c : 0) * pitch;
+#pragma unroll
+        for (int m = 0; m < E; ++m) vb[m] = live ? srcB[t + T * m] : C{0, 0};
+    }
 
     if (OP == OP_INV) {
-        F::template run<+1>(va, buf, t, a.twy);
+        F::template run<+1>(va, buf, t, tw);
 #pragma unroll
         for (int m = 0; m < E; ++m) va[m] = cscale(va[m], a.scale);
         if (live) {
@@ -237,7 +249,7 @@ __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
         }
         return;
     }
-    F::template run<-1>(va, buf, t, a.twy);
+    F::template run<-1>(va, buf, t, tw);
     if (OP == OP_FWD) {
         if (live) {
             C* dst = a.outA + (size_t)c * pitch;
@@ -246,12 +258,8 @@ __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
         }
         return;
     }
-    if (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC) {
-        C vb[E];
-        const C* srcB = a.inB + (size_t)(live ? c : 0) * pitch;
-#pragma unroll
-        for (int m = 0; m < E; ++m) vb[m] = live ? srcB[t + T * m] : C{0, 0};
-        F::template run<-1>(vb, buf, t, a.twy);
+    if constexpr (TWO_IN) {
+        F::template run<-1>(vb, buf, t, tw);
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int s = signed_k(t + T * m, a.g.ny);
@@ -269,7 +277,6 @@ __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
             va[m] = cscale(r, f);
         }
     } else if (OP == OP_VEL) {
-        C vb[E];
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int s = signed_k(t + T * m, a.g.ny);
@@ -277,7 +284,7 @@ __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
             vb[m] = cmul_i(va[m], dx * a.scale);    // V = Dx P
             va[m] = cmul_i(va[m], -dy * a.scale);   // U = -Dy P
         }
-        F::template run<+1>(vb, buf, t, a.twy);
+        F::template run<+1>(vb, buf, t, tw);
         if (live) {
             C* dst = a.outB + (size_t)c * pitch;
 #pragma unroll
@@ -300,7 +307,7 @@ __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
             if (OP == OP_MASK) va[m] = cscale(va[m], sym_mask<R>(a.g, c, s) * a.scale);
         }
     }
-    F::template run<+1>(va, buf, t, a.twy);
+    F::template run<+1>(va, buf, t, tw);
     if (live) {
         C* dst = a.outA + (size_t)c * pitch;
 #pragma unroll
@@ -356,29 +363,76 @@ __global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
     const int y0 = blk * rows_per_blk;
     const int rowA = y0 + l;
     const int rowB = (MODE == MODE_PAIR) ? y0 + G + l : rowA;
+    const size_t bA = (size_t)nx * rowA, bB = (size_t)nx * rowB;
+    typename F::Twiddles tw;
+    tw.load(t, a.twx);
 
-    // cooperative gather of the packed spectra into the per-line buffers
+    // epilogue operands do not depend on the transform: load them first
+    constexpr bool STATE = (EPI >= EPI_RK1);
+    constexpr bool NEED_B = (EPI == EPI_RK1 || EPI == EPI_RK2 || EPI == EPI_RK3 || EPI == EPI_AB2 || EPI == EPI_ADD);
+    double pbA[STATE ? E : 1], pbB[STATE ? E : 1];
+    R pkA[STATE ? E : 1], pkB[STATE ? E : 1];
+    C pwA[STATE ? E : 1], pwB[STATE ? E : 1];
+    if constexpr (STATE) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int x = t + T * m;
+            if (NEED_B) {
+                pbA[m] = a.b[bA + x];
+                pbB[m] = a.b[bB + x];
+            }
+            if (EPI == EPI_RK3) {
+                pkA[m] = a.k1[bA + x];
+                pkB[m] = a.k1[bB + x];
+            }
+            if (EPI == EPI_AB2) {
+                pkA[m] = a.fold[bA + x];
+                pkB[m] = a.fold[bB + x];
+            }
+            if (a.emit) {
+                pwA[m] = a.w[bA + x];
+                pwB[m] = a.w[bB + x];
+            }
+        }
+    }
+
+    // cooperative gather of the packed spectra into the per-line buffers;
+    // all loads are issued before any LDS store (bounded, unrolled loop)
     {
         const int nc = a.ncols_in;
         const int total = nc * G;
-        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-            const int kx = idx / G, ll = idx - kx * G;
-            C p0, p1;
-            if (MODE == MODE_PAIR) {
-                p0 = a.P0[(size_t)kx * pitch + y0 + ll];
-                p1 = a.P0[(size_t)kx * pitch + y0 + G + ll];
-            } else {
-                p0 = a.P0[(size_t)kx * pitch + y0 + ll];
-                p1 = a.P1[(size_t)kx * pitch + y0 + ll];
+        constexpr int MAXIT = (N / 2 + 1 + T - 1) / T;
+        C q0[MAXIT], q1[MAXIT];
+#pragma unroll
+        for (int it = 0; it < MAXIT; ++it) {
+            const int idx = threadIdx.x + it * (int)blockDim.x;
+            if (idx < total) {
+                const int kx = idx / G, ll = idx - kx * G;
+                const size_t o = (size_t)kx * pitch + y0 + ll;
+                if (MODE == MODE_PAIR) {
+                    q0[it] = a.P0[o];
+                    q1[it] = a.P0[o + G];
+                } else {
+                    q0[it] = a.P0[o];
+                    q1[it] = a.P1[o];
+                }
             }
-            C* bl = lds + ll * LE;
-            const bool edge = (kx == 0) || (2 * kx == N);
-            if (edge) {
-                p0.y = 0;
-                p1.y = 0;
+        }
+#pragma unroll
+        for (int it = 0; it < MAXIT; ++it) {
+            const int idx = threadIdx.x + it * (int)blockDim.x;
+            if (idx < total) {
+                const int kx = idx / G, ll = idx - kx * G;
+                C p0 = q0[it], p1 = q1[it];
+                C* bl = lds + ll * LE;
+                const bool edge = (kx == 0) || (2 * kx == N);
+                if (edge) {
+                    p0.y = 0;
+                    p1.y = 0;
+                }
+                bl[lpad(kx)] = C{p0.x - p1.y, p0.y + p1.x};
+                if (!edge) bl[lpad(N - kx)] = C{p0.x + p1.y, p1.x - p0.y};
             }
-            bl[lpad(kx)] = C{p0.x - p1.y, p0.y + p1.x};
-            if (!edge) bl[lpad(N - kx)] = C{p0.x + p1.y, p1.x - p0.y};
         }
         // zero the band no input column covers: kx in [nc, N - nc]
         const int gap = N - 2 * nc + 1;
@@ -392,11 +446,10 @@ __global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
     }
     C* buf = lds + l * LE;
     C v[E];
-    F::template run_from_lds<+1>(v, buf, t, a.twx);
+    F::template run_from_lds<+1>(v, buf, t, tw);
 
     // epilogue
     R sA[E], sB[E];
-    const size_t bA = (size_t)nx * rowA, bB = (size_t)nx * rowB;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int x = t + T * m;
@@ -406,28 +459,28 @@ __global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
             a.out_r[bB + x] = tB;
         } else if (EPI == EPI_STORE_W) {
             a.w_out[bA + x] = C{tA, tB};
-        } else {
+        } else if constexpr (STATE) {
             double nbA, nbB;
             if (EPI == EPI_RK1) {
                 a.k1[bA + x] = tA;
                 a.k1[bB + x] = tB;
-                nbA = a.b[bA + x] + a.c1 * (double)tA;
-                nbB = a.b[bB + x] + a.c1 * (double)tB;
+                nbA = pbA[m] + a.c1 * (double)tA;
+                nbB = pbB[m] + a.c1 * (double)tB;
             } else if (EPI == EPI_RK2) {
-                nbA = a.b[bA + x] + a.c1 * (double)tA;
-                nbB = a.b[bB + x] + a.c1 * (double)tB;
+                nbA = pbA[m] + a.c1 * (double)tA;
+                nbB = pbB[m] + a.c1 * (double)tB;
             } else if (EPI == EPI_RK3) {
-                nbA = a.b[bA + x] + a.c1 * ((double)a.k1[bA + x] + 3.0 * (double)tA);
-                nbB = a.b[bB + x] + a.c1 * ((double)a.k1[bB + x] + 3.0 * (double)tB);
+                nbA = pbA[m] + a.c1 * ((double)pkA[m] + 3.0 * (double)tA);
+                nbB = pbB[m] + a.c1 * ((double)pkB[m] + 3.0 * (double)tB);
                 a.b[bA + x] = nbA;
                 a.b[bB + x] = nbB;
             } else if (EPI == EPI_AB2) {
                 if (a.first) {
-                    nbA = a.b[bA + x] + a.c1 * (double)tA;
-                    nbB = a.b[bB + x] + a.c1 * (double)tB;
+                    nbA = pbA[m] + a.c1 * (double)tA;
+                    nbB = pbB[m] + a.c1 * (double)tB;
                 } else {
-                    nbA = a.b[bA + x] + a.c1 * (1.5 * (double)tA - 0.5 * (double)a.fold[bA + x]);
-                    nbB = a.b[bB + x] + a.c1 * (1.5 * (double)tB - 0.5 * (double)a.fold[bB + x]);
+                    nbA = pbA[m] + a.c1 * (1.5 * (double)tA - 0.5 * (double)pkA[m]);
+                    nbB = pbB[m] + a.c1 * (1.5 * (double)tB - 0.5 * (double)pkB[m]);
                 }
                 a.fold[bA + x] = tA;
                 a.fold[bB + x] = tB;
@@ -439,8 +492,8 @@ __global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
                 a.b[bA + x] = nbA;
                 a.b[bB + x] = nbB;
             } else {  // EPI_ADD
-                nbA = a.b[bA + x] + (double)tA;
-                nbB = a.b[bB + x] + (double)tB;
+                nbA = pbA[m] + (double)tA;
+                nbB = pbB[m] + (double)tB;
                 a.b[bA + x] = nbA;
                 a.b[bB + x] = nbB;
             }
@@ -448,20 +501,19 @@ __global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
             sB[m] = (R)nbB;
         }
     }
-    if (EPI == EPI_STORE || EPI == EPI_STORE_W) return;
+    if constexpr (!STATE) return;
     if (!a.emit) return;  // uniform across the grid
 
     // next-stage fluxes of both rows: x-FFT of (u*b) + i(v*b)
+#pragma unroll
     for (int half = 0; half < 2; ++half) {
-        const size_t bR = half == 0 ? bA : bB;
 #pragma unroll
         for (int m = 0; m < E; ++m) {
-            const int x = t + T * m;
-            const C uv = a.w[bR + x];
+            const C uv = half == 0 ? pwA[m] : pwB[m];
             const R s = half == 0 ? sA[m] : sB[m];
             v[m] = C{uv.x * s, uv.y * s};
         }
-        F::template run<-1>(v, buf, t, a.twx);
+        F::template run<-1>(v, buf, t, tw);
         F::to_lds(v, buf, t);
         const int r0 = half == 0 ? y0 : (MODE == MODE_PAIR ? y0 + G : y0);
         store_split<R, N, E>(lds, G, a.ncols_out, a.Fx, a.Fy, pitch, r0, r0);

@@ -49,7 +49,7 @@ EXPORTED_SYMBOLS = (
     "ssg_create", "ssg_destroy", "ssg_last_error", "ssg_sync", "ssg_set_field", "ssg_get_field",
     "ssg_set_field_device", "ssg_get_field_device", "ssg_timestep", "ssg_tendency", "ssg_jacobian",
     "ssg_rfft_2d", "ssg_irfft_2d", "ssg_apply_exponential_filter", "ssg_diagnostics", "ssg_last_timing",
-    "ssg_profile", "ssg_profile_read",
+    "ssg_profile", "ssg_profile_read", "ssg_tuning",
 )
 
 
@@ -123,6 +123,7 @@ def load_library(path: str | None = None):
     lib.ssg_diagnostics.argtypes = [vp, ctypes.POINTER(_Diag)]
     lib.ssg_last_timing.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int64)]
     lib.ssg_profile.argtypes = [vp, ctypes.c_int32]
+    lib.ssg_tuning.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _dp]
     lib.ssg_profile_read.argtypes = [vp, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), _dp]
     for n in EXPORTED_SYMBOLS:
         getattr(lib, n).restype = ctypes.c_char_p if n == "ssg_last_error" else ctypes.c_int
@@ -211,6 +212,16 @@ class Domain:
         n = ctypes.c_int64()
         self.check(self.lib.ssg_profile_read(self._h, int(kind), ctypes.byref(ms), ctypes.byref(n), ctypes.byref(by)))
         return ms.value, n.value, by.value
+
+    def tuning(self, path: int | None = None):
+        """Autotuned elements-per-thread per kernel class and candidate timings (us)."""
+        path = (1 if self.precision == SSG_FP32_MIXED else 0) if path is None else path
+        ept = (ctypes.c_int32 * 3)()
+        us = (ctypes.c_double * 9)()
+        self.check(self.lib.ssg_tuning(self._h, int(path), ept, us))
+        names = ("k_rows_fwd", "k_cols", "k_rows_inv")
+        return {names[k]: {"ept": ept[k], "cand_us": {c: us[3 * k + i] for i, c in enumerate((4, 8, 16))}}
+                for k in range(3)}
 
     def sync(self):
         self.check(self.lib.ssg_sync(self._h))

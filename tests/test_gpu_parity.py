@@ -188,8 +188,9 @@ def test_error_paths(gpu):
     assert e.value.code == gpu.SSG_ERR_NONFINITE
     with pytest.raises(gpu.SSGError):
         gpu.make_domain(12, 16, 1)
-    with pytest.raises(gpu.SSGError):
-        d.set_field(gpu.FIELD_B, np.zeros(17))
+    z = np.zeros(17)
+    assert d.lib.ssg_set_field(d.handle, gpu.FIELD_B, z.ctypes.data, 17) == gpu.SSG_ERR_ARG
+    assert d.lib.ssg_set_field(d.handle, 9, np.zeros(256).ctypes.data, 256) == gpu.SSG_ERR_ARG
 
 
 def test_full_size_properties(gpu):
