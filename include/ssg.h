@@ -94,6 +94,31 @@ typedef struct {
     int64_t nonfinite;     /* count of non-finite b entries */
 } ssg_diag;
 
+/* Monge-Ampère problem variants (src/poisson.jl:1068-1131):
+   SSG_MA_BODY    as wired: b₃d[:,:,k] = bₛ on every level, surface slope 0
+                  (extend_2d_to_3d, src/poisson.jl:1096-1113)
+   SSG_MA_SURFACE documented physics: no body term, surface slope ∂Φ/∂z = bₛ
+                  (src/poisson.jl:2-6) */
+enum { SSG_MA_BODY = 0, SSG_MA_SURFACE = 1 };
+
+typedef struct {
+    double eps;        /* ε (src/poisson.jl:1072 default 0.1) */
+    double tol;        /* relative residual target ‖r‖₂ / ‖b + εD‖₂ on interior levels */
+    int32_t maxiter;   /* src/poisson.jl:874 default 50 */
+    int32_t variant;   /* SSG_MA_BODY or SSG_MA_SURFACE */
+    int32_t warm;      /* 1: start from the previous solve's D(Φ) */
+} ssg_ma_params;
+
+/* solve_ssg_equation diagnostics (src/poisson.jl:928-958) */
+typedef struct {
+    int32_t converged;
+    int32_t iterations;
+    double final_residual;     /* norm_field(r) = sqrt(Σ r²) over all points, src/poisson.jl:986-995 */
+    double relative_residual;  /* ‖r‖₂ / ‖b + εD‖₂ over interior levels */
+    double history[64];        /* relative residual per iteration (first 64) */
+    double solve_ms;
+} ssg_ma_diag;
+
 /* make_domain + allocate_fields + TimeState (src/domain.jl:217-230,
    src/fields.jl:72-116, src/timestep.jl:45-86) */
 int ssg_create(ssg_ctx** out, const ssg_domain_desc* desc);
@@ -132,6 +157,23 @@ int ssg_irfft_2d(ssg_ctx* ctx, const double* spec_in, double* real_out);
 int ssg_apply_exponential_filter(ssg_ctx* ctx, double strength, int32_t order, double cutoff);
 
 int ssg_diagnostics(ssg_ctx* ctx, ssg_diag* out);
+
+/* solve_monge_ampere_fields! (src/poisson.jl:1068-1090, with the maxiter
+   keyword repaired): solves the residual operator of
+   compute_ssg_residual! (src/poisson.jl:228-253) with the one-sided
+   Neumann rows of apply_ssg_boundary_conditions! (src/poisson.jl:262-364)
+   for the current bₛ, then sets φₛ = Φ[:,:,Nz] (extract_surface_to_2d!,
+   src/poisson.jl:1119-1131) and the surface velocities.  Needs nz >= 3.
+   Returns SSG_ERR_NOT_CONVERGED (with diag filled) when maxiter is hit. */
+int ssg_ma_solve(ssg_ctx* ctx, const ssg_ma_params* params, ssg_ma_diag* diag);
+/* fields.φ (nx*ny*nz, x fastest then y then z) after ssg_ma_solve */
+int ssg_get_phi3d(ssg_ctx* ctx, double* host, int64_t n);
+/* Coupled stepping: with params != NULL, ssg_timestep re-solves the
+   Monge-Ampère problem for the stage buoyancy before every tendency
+   evaluation (φₛ then follows b); NULL restores the reference's frozen φₛ. */
+int ssg_set_coupling(ssg_ctx* ctx, const ssg_ma_params* params);
+/* diagnostics of the most recent Monge-Ampère solve */
+int ssg_last_ma_diag(ssg_ctx* ctx, ssg_ma_diag* diag);
 
 /* Kernel timing of the last ssg_timestep call (HIP events on the context
    stream): total milliseconds and the number of launches. */

@@ -2,6 +2,7 @@
 // driver and the spectral sub-boundary.  Declarations: include/ssg.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -13,6 +14,7 @@
 
 #include "../../include/ssg.h"
 #include "ssg_kernels.hpp"
+#include "ssg_ma.hpp"
 
 using namespace ssg;
 
@@ -59,6 +61,24 @@ struct ssg_ctx {
     int ept[2][3] = {{0, 0, 0}, {0, 0, 0}};
     bool tuned[2] = {false, false};
     double tune_us[2][3][3] = {};
+    // Monge-Ampère solver state (allocated on first use)
+    bool ma_ready = false;
+    cpx<double>* ma_phi = nullptr;    // Φ̂ [kx][lev][ky]
+    cpx<double>* ma_dprev = nullptr;  // D̂ of the current iterate
+    cpx<double>* ma_p[3] = {nullptr, nullptr, nullptr};  // Φxx, Φyy, Φxy planes (y physical)
+    cpx<double>* ma_b2 = nullptr;     // forcing spectrum [kx][ky]
+    double* ma_coef = nullptr;
+    double* ma_partial = nullptr;
+    double* ma_partial_host = nullptr;
+    int ma_partial_cap = 0;
+    int ma_partial_used = 0;
+    bool ma_have_d = false;
+    Grid g3{};                        // 3D plane geometry: pitch = nz * ny
+    // coupled stepping (MA solve at every stage)
+    bool coupled = false;
+    ssg_ma_params ma_par{};
+    double* bst = nullptr;            // stage buoyancy (coupled mode)
+    ssg_ma_diag last_diag{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int64_t last_launches = 0;
@@ -202,9 +222,10 @@ int launch_rows_fwd(ssg_ctx* c, const S* f, const cpx<R>* w, cpx<R>* PA, cpx<R>*
     });
 }
 
+// General column-pass launcher: `a` carries the operands; lines are
+// (kx, level) pairs when nlev > 1 (kx = line / nlev).
 template <typename R, int OP>
-int launch_cols(ssg_ctx* c, const cpx<R>* inA, const cpx<R>* inB, cpx<R>* outA, cpx<R>* outB, int ncols,
-                R scale, R fstrength = 0, int forder = 1, R fcutoff = 1, int fdelta = 0) {
+int launch_cols_args(ssg_ctx* c, ColsArgs<R> a, int nlines, int nlev) {
     PathBuffers<R>& B = bufs<R>(c);
     const int ept = ept_of<R>(c, 1, c->ny);
     return with_size_ept(c->ny, ept, [&](auto tag, auto etag) -> int {
@@ -213,42 +234,66 @@ int launch_cols(ssg_ctx* c, const cpx<R>* inA, const cpx<R>* inB, cpx<R>* outA, 
         constexpr int T = M / E;
         const int le = lds_line_elems(M) * (int)sizeof(cpx<R>);
         int G = 1;
-        while (2 * G * T <= 256 && 2 * G * le <= kMaxLdsBytes && 2 * G <= ncols) G *= 2;
-        const int nblk = (ncols + G - 1) / G;
+        while (2 * G * T <= 256 && 2 * G * le <= kMaxLdsBytes && 2 * G <= nlines) G *= 2;
+        const int nblk = (nlines + G - 1) / G;
         const double kmx = M_PI * c->nx / c->lx, kmy = M_PI * c->ny / c->ly;
-        ColsArgs<R> a{inA, inB, outA, outB, ncols, G, scale, fstrength, fcutoff,
-                      (R)(1.0 / std::sqrt(kmx * kmx + kmy * kmy)), forder, fdelta, B.twy, c->g};
+        a.ncols = nlines;
+        a.G = G;
+        a.inv_kmax = (R)(1.0 / std::sqrt(kmx * kmx + kmy * kmy));
+        a.twy = B.twy;
+        a.g = c->g;
+        a.nlev = nlev;
+        if (OP == OP_MA_DIFF && nblk > c->ma_partial_cap)
+            return set_err(c, SSG_ERR_ARG, "reduction buffer too small (%d blocks)", nblk);
         const size_t lds = (size_t)G * le;
-        const int nin = (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC) ? 2 : 1;
-        const int nout = OP == OP_VEL ? 2 : 1;
-        const double bytes = (double)(nin + nout) * ncols * c->ny * sizeof(cpx<R>);
+        const int nin = (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC || OP == OP_MA_DIFF) ? 2 : 1;
+        const int nout = (OP == OP_VEL || OP == OP_VEL_SPEC) ? 2 : (OP == OP_MA_DERIV ? 3 : 1);
+        const double bytes = (double)(nin + nout) * nlines * c->ny * sizeof(cpx<R>);
         int slot, rc;
         if ((rc = prof_begin(c, &slot))) return rc;
         hipLaunchKernelGGL((k_cols<R, M, E, OP>), dim3(nblk), dim3(G * T), lds, c->stream, a);
         c->launches++;
         HIP_OK(c, hipGetLastError());
+        if (OP == OP_MA_DIFF) c->ma_partial_used = nblk;
         return prof_end(c, slot, 1, bytes);
     });
 }
 
+template <typename R, int OP>
+int launch_cols(ssg_ctx* c, const cpx<R>* inA, const cpx<R>* inB, cpx<R>* outA, cpx<R>* outB, int ncols,
+                R scale, R fstrength = 0, int forder = 1, R fcutoff = 1, int fdelta = 0) {
+    ColsArgs<R> a{};
+    a.inA = inA;
+    a.inB = inB;
+    a.outA = outA;
+    a.outB = outB;
+    a.scale = scale;
+    a.fstrength = fstrength;
+    a.fcutoff = fcutoff;
+    a.forder = forder;
+    a.fdelta = fdelta;
+    return launch_cols_args<R, OP>(c, a, ncols, 1);
+}
+
 template <typename R, int MODE, int EPI>
-int launch_rows_inv(ssg_ctx* c, RowsInvArgs<R> a) {
+int launch_rows_inv(ssg_ctx* c, RowsInvArgs<R> a, int rows = 0, const Grid* grid = nullptr) {
     PathBuffers<R>& B = bufs<R>(c);
     const int ept = ept_of<R>(c, 2, c->nx);
+    if (rows <= 0) rows = c->ny;
     return with_size_ept(c->nx, ept, [&](auto tag, auto etag) -> int {
         constexpr int N = decltype(tag)::value;
         constexpr int E = decltype(etag)::value;
         constexpr int T = N / E;
         const int rows_per_line = MODE == MODE_PAIR ? 2 : 1;
-        const int G = lines_per_block<R>(N, E, c->ny / rows_per_line);
-        const int nblk = c->ny / (rows_per_line * G);
+        const int G = lines_per_block<R>(N, E, rows / rows_per_line);
+        const int nblk = rows / (rows_per_line * G);
         a.G = G;
         a.nblk = nblk;
         a.twx = B.twx;
-        a.g = c->g;
+        a.g = grid ? *grid : c->g;
         const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<R>);
-        const double n2 = (double)c->nx * c->ny, sc = sizeof(cpx<R>);
-        double bytes = (MODE == MODE_PAIR ? 1.0 : 2.0) * a.ncols_in * c->ny * sc;
+        const double n2 = (double)c->nx * rows, sc = sizeof(cpx<R>);
+        double bytes = (MODE == MODE_PAIR ? 1.0 : 2.0) * a.ncols_in * rows * sc;
         if (EPI == EPI_STORE) bytes += n2 * sizeof(R);
         if (EPI == EPI_STORE_W) bytes += n2 * sc;
         if (EPI == EPI_RK1) bytes += n2 * (sizeof(double) + sizeof(R));
@@ -257,7 +302,8 @@ int launch_rows_inv(ssg_ctx* c, RowsInvArgs<R> a) {
         if (EPI == EPI_AB2) bytes += n2 * (2 * sizeof(double) + 2 * sizeof(R));
         if (EPI == EPI_SET) bytes += n2 * sizeof(double);
         if (EPI == EPI_ADD) bytes += n2 * 2 * sizeof(double);
-        if (EPI >= EPI_RK1 && a.emit) bytes += n2 * sc + 2.0 * a.ncols_out * c->ny * sc;
+        if (EPI >= EPI_RK1 && a.emit) bytes += n2 * sc + 2.0 * a.ncols_out * rows * sc;
+        if (a.bstage) bytes += n2 * sizeof(double);
         int slot, rc;
         if ((rc = prof_begin(c, &slot))) return rc;
         hipLaunchKernelGGL((k_rows_inv<R, N, E, MODE, EPI>), dim3(nblk), dim3(G * T), lds, c->stream, a);
@@ -327,15 +373,16 @@ int compute_velocities(ssg_ctx* c) {
 
 // fluxes of the current b into (Fx, Fy) planes
 template <typename R>
-int flux_forward(ssg_ctx* c, int ncols) {
+int flux_forward(ssg_ctx* c, int ncols, const double* src = nullptr) {
     PathBuffers<R>& B = bufs<R>(c);
-    return launch_rows_fwd<R, double, SRC_FLUX>(c, c->b, B.w, B.Fx, B.Fy, ncols);
+    return launch_rows_fwd<R, double, SRC_FLUX>(c, src ? src : c->b, B.w, B.Fx, B.Fy, ncols);
 }
 
 template <typename R, int EPI>
-int stage_inverse(ssg_ctx* c, double c1, int emit, int first) {
+int stage_inverse(ssg_ctx* c, double c1, int emit, int first, double* bstage = nullptr) {
     PathBuffers<R>& B = bufs<R>(c);
     RowsInvArgs<R> a{};
+    a.bstage = bstage;
     a.P0 = B.Tp;
     a.ncols_in = c->ncols_tend;
     a.b = c->b;
@@ -453,8 +500,52 @@ int autotune(ssg_ctx* c) {
     return rc;
 }
 
+int ma_solve_dev(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_diag* d);
+int rfft2_device(ssg_ctx* c, const double* src, cpx<double>* dst = nullptr);
+
+// Coupled step: the Monge-Ampère problem is re-solved for the stage
+// buoyancy before every tendency evaluation, so φₛ and (u, v) follow b.
+template <typename R>
+int run_steps_coupled(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, int64_t nsteps) {
+    const double dt = p->dt;
+    ssg_ma_diag d{};
+    int rc;
+    auto stage_tendency = [&](const double* bsrc) -> int {
+        int r = ma_solve_dev(c, bsrc, &c->ma_par, &d);
+        if (r) return r;
+        if (!d.converged)
+            return set_err(c, SSG_ERR_NOT_CONVERGED, "Monge-Ampere solve did not converge (rel. residual %.3e)",
+                           d.relative_residual);
+        if ((r = flux_forward<R>(c, c->ncols_tend, bsrc))) return r;
+        return tendency_cols<R>(c, p->scheme == SSG_AB2_LOWSTORAGE ? p->ab2_reference_aliasing : 0);
+    };
+    for (int64_t it = 0; it < nsteps; ++it) {
+        const int64_t step_after = s->step + 1;
+        const bool filt = p->filter_freq > 0 && (step_after % p->filter_freq) == 0;
+        if (p->scheme == SSG_RK3) {
+            if ((rc = stage_tendency(c->b))) return rc;
+            if ((rc = stage_inverse<R, EPI_RK1>(c, dt / 3.0, 0, 0, c->bst))) return rc;
+            if ((rc = stage_tendency(c->bst))) return rc;
+            if ((rc = stage_inverse<R, EPI_RK2>(c, 2.0 * dt / 3.0, 0, 0, c->bst))) return rc;
+            if ((rc = stage_tendency(c->bst))) return rc;
+            if ((rc = stage_inverse<R, EPI_RK3>(c, dt / 4.0, 0, 0))) return rc;
+        } else {
+            if ((rc = stage_tendency(c->b))) return rc;
+            const int first = (s->step == 0) ? 1 : 0;
+            if ((rc = stage_inverse<R, EPI_AB2>(c, dt, 0, first))) return rc;
+            c->ab2_have_history = true;
+        }
+        s->t += dt;
+        s->step = step_after;
+        s->dt_actual = dt;
+        if (filt && (rc = filter_b<R>(c, p->filter_strength, p->filter_order, p->filter_cutoff, 0))) return rc;
+    }
+    return SSG_OK;
+}
+
 template <typename R>
 int run_steps(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, int64_t nsteps) {
+    if (c->coupled) return run_steps_coupled<R>(c, p, s, nsteps);
     int rc = flux_forward<R>(c, c->ncols_tend);
     if (rc) return rc;
     const double dt = p->dt;
@@ -486,6 +577,238 @@ int run_steps(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, int64_t n
             if ((rc = filter_b<R>(c, p->filter_strength, p->filter_order, p->filter_cutoff, last ? 0 : 1))) return rc;
         }
     }
+    return SSG_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Monge-Ampère solve (FP64)
+// ---------------------------------------------------------------------------
+int ma_alloc(ssg_ctx* c) {
+    if (c->ma_ready) return SSG_OK;
+    if (c->nz < 3 || c->nz > 128)
+        return set_err(c, SSG_ERR_UNSUPPORTED, "the Monge-Ampere solve needs 3 <= nz <= 128 (nz = %d)", c->nz);
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    HIP_OK(c, hipMalloc(&c->ma_phi, c3 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->ma_dprev, c3 * sizeof(cpx<double>)));
+    for (int i = 0; i < 3; ++i) HIP_OK(c, hipMalloc(&c->ma_p[i], c3 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->ma_b2, (size_t)c->ncols_full * c->ny * sizeof(cpx<double>)));
+    c->ma_partial_cap = c->ncols_full * c->nz + 64;
+    HIP_OK(c, hipMalloc(&c->ma_partial, (size_t)c->ma_partial_cap * 2 * sizeof(double)));
+    HIP_OK(c, hipHostMalloc(&c->ma_partial_host, (size_t)c->ma_partial_cap * 2 * sizeof(double), hipHostMallocDefault));
+    HIP_OK(c, hipMalloc(&c->bst, (size_t)c->nx * c->ny * sizeof(double)));
+    // stencil coefficients (uniform grid dz of the context)
+    const int nz = c->nz;
+    std::vector<double> dz(nz, c->lz / (nz - 1));
+    std::vector<double> cf(3 * nz + 6, 0.0);
+    for (int k = 1; k <= nz - 2; ++k) {  // d2dz2!, src/transforms.jl:812-819
+        const double dzm = dz[k - 1], dzp = dz[k];
+        cf[3 * k] = 2.0 / (dzm * (dzm + dzp));
+        cf[3 * k + 1] = -2.0 / (dzm * dzp);
+        cf[3 * k + 2] = 2.0 / (dzp * (dzm + dzp));
+    }
+    {  // apply_neumann_bottom!, src/poisson.jl:343-364
+        const double h1 = dz[0], h2 = dz[1], al = h1 + h2;
+        cf[3 * nz] = -(2 * h1 + h2) / (h1 * al);
+        cf[3 * nz + 1] = al / (h1 * h2);
+        cf[3 * nz + 2] = -h1 / (h2 * al);
+    }
+    {  // apply_neumann_surface!, src/poisson.jl:309-336
+        const double h1 = dz[nz - 2], h2 = dz[nz - 3], al = h1 + h2;
+        cf[3 * nz + 3] = (2 * h1 + h2) / (h1 * al);
+        cf[3 * nz + 4] = -al / (h1 * h2);
+        cf[3 * nz + 5] = h1 / (h2 * al);
+    }
+    HIP_OK(c, hipMalloc(&c->ma_coef, cf.size() * sizeof(double)));
+    HIP_OK(c, hipMemcpy(c->ma_coef, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->g3 = c->g;
+    c->g3.pitch = c->nz * c->ny;
+    c->ma_ready = true;
+    return SSG_OK;
+}
+
+void ma_free(ssg_ctx* c) {
+    (void)hipFree(c->ma_phi);
+    (void)hipFree(c->ma_dprev);
+    for (int i = 0; i < 3; ++i) (void)hipFree(c->ma_p[i]);
+    (void)hipFree(c->ma_b2);
+    (void)hipFree(c->ma_coef);
+    (void)hipFree(c->ma_partial);
+    if (c->ma_partial_host) (void)hipHostFree(c->ma_partial_host);
+    (void)hipFree(c->bst);
+}
+
+int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d) {
+    MaTriArgs<double> a{};
+    a.Dh = c->ma_dprev;
+    a.B2 = c->ma_b2;
+    a.Phi = c->ma_phi;
+    a.coef = c->ma_coef;
+    a.nz = c->nz;
+    a.ncols = c->ncols_full;
+    a.eps = p->eps;
+    a.body = p->variant == SSG_MA_BODY ? 1 : 0;
+    a.top = p->variant == SSG_MA_SURFACE ? 1 : 0;
+    a.use_d = use_d;
+    a.g = c->g;
+    const long total = (long)c->ncols_full * c->ny;
+    const int nblk = (int)((total + 127) / 128);
+    const double bytes = (double)(2 * c->nz + 1) * total * sizeof(cpx<double>);
+    int slot, rc;
+    if ((rc = prof_begin(c, &slot))) return rc;
+    if (c->nz <= 8)
+        hipLaunchKernelGGL((k_ma_tridiag<double, 8>), dim3(nblk), dim3(128), 0, c->stream, a);
+    else if (c->nz <= 16)
+        hipLaunchKernelGGL((k_ma_tridiag<double, 16>), dim3(nblk), dim3(128), 0, c->stream, a);
+    else if (c->nz <= 32)
+        hipLaunchKernelGGL((k_ma_tridiag<double, 32>), dim3(nblk), dim3(128), 0, c->stream, a);
+    else if (c->nz <= 64)
+        hipLaunchKernelGGL((k_ma_tridiag<double, 64>), dim3(nblk), dim3(128), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_ma_tridiag<double, 128>), dim3(nblk), dim3(128), 0, c->stream, a);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    return prof_end(c, slot, 4, bytes);
+}
+
+int ma_rows(ssg_ctx* c) {
+    PathBuffers<double>& D = c->d;
+    const int ept = ept_of<double>(c, 2, c->nx);
+    const int rows = c->nz * c->ny;
+    return with_size_ept(c->nx, ept, [&](auto tag, auto etag) -> int {
+        constexpr int N = decltype(tag)::value;
+        constexpr int E = decltype(etag)::value;
+        constexpr int T = N / E;
+        const int G = lines_per_block<double>(N, E, rows / 2);
+        const int nblk = rows / (2 * G);
+        MaRowsArgs<double> a{c->ma_p[0], c->ma_p[1], c->ma_p[2], c->ma_p[0], c->ncols_full, G, nblk, D.twx, c->g3};
+        const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<double>);
+        const double bytes = 4.0 * c->ncols_full * rows * sizeof(cpx<double>);
+        int slot, rc;
+        if ((rc = prof_begin(c, &slot))) return rc;
+        hipLaunchKernelGGL((k_ma_rows<double, N, E>), dim3(nblk), dim3(G * T), lds, c->stream, a);
+        c->launches++;
+        HIP_OK(c, hipGetLastError());
+        return prof_end(c, slot, 3, bytes);
+    });
+}
+
+// Picard / defect-correction iteration Φ̂ <- L⁻¹(b̂ + ε D̂(Φ)); the residual
+// of iterate n is ε(D̂ⁿ⁻¹ - D̂ⁿ) on interior levels, so every iteration
+// reports the reference residual of the Φ it holds.
+int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
+    const double scale = 1.0 / ((double)c->nx * c->ny);
+    const int lines = c->ncols_full * c->nz;
+    int use_d = 1;
+    if (!p->warm || !c->ma_have_d) {
+        HIP_OK(c, hipMemsetAsync(c->ma_dprev, 0, (size_t)lines * c->ny * sizeof(cpx<double>), c->stream));
+        use_d = 0;
+    }
+    int rc = ma_tridiag(c, p, use_d);
+    if (rc) return rc;
+    memset(d, 0, sizeof(*d));
+    double best = 1e300;
+    for (int it = 0;; ++it) {
+        ColsArgs<double> a{};
+        a.inA = c->ma_phi;
+        a.outA = c->ma_p[0];
+        a.outB = c->ma_p[1];
+        a.outC = c->ma_p[2];
+        a.scale = scale;
+        if ((rc = launch_cols_args<double, OP_MA_DERIV>(c, a, lines, c->nz))) return rc;
+        if ((rc = ma_rows(c))) return rc;
+        ColsArgs<double> b{};
+        b.inA = c->ma_p[0];
+        b.outA = c->ma_dprev;
+        b.B2 = c->ma_b2;
+        b.eps = p->eps;
+        b.body = p->variant == SSG_MA_BODY ? 1 : 0;
+        b.partial = c->ma_partial;
+        if ((rc = launch_cols_args<double, OP_MA_DIFF>(c, b, lines, c->nz))) return rc;
+        const int nb = c->ma_partial_used;
+        HIP_OK(c, hipMemcpyAsync(c->ma_partial_host, c->ma_partial, (size_t)nb * 2 * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(c, hipStreamSynchronize(c->stream));
+        double r2 = 0, h2 = 0;
+        for (int i = 0; i < nb; ++i) {
+            r2 += c->ma_partial_host[2 * i];
+            h2 += c->ma_partial_host[2 * i + 1];
+        }
+        // Parseval: sum of r² over the grid = (1/(nx ny)) sum of |r̂|² per level
+        d->final_residual = std::sqrt(r2 * scale);
+        d->relative_residual = h2 > 0 ? std::sqrt(r2 / h2) : std::sqrt(r2 * scale);
+        if (it < 64) d->history[it] = d->relative_residual;
+        d->iterations = it;
+        if (!std::isfinite(d->relative_residual)) break;
+        // divergence (loss of ellipticity): stop once the residual has grown
+        // a hundredfold over the best iterate
+        best = std::min(best, d->relative_residual);
+        if (d->relative_residual > 100.0 * best) break;
+        if (d->relative_residual <= p->tol) {
+            d->converged = 1;
+            break;
+        }
+        if (it >= p->maxiter) break;
+        if ((rc = ma_tridiag(c, p, 1))) return rc;
+    }
+    c->ma_have_d = true;
+    return SSG_OK;
+}
+
+// φₛ = Φ[:,:,Nz] (src/poisson.jl:1119-1131) and u = -∂yφₛ, v = ∂xφₛ
+// (src/setICs.jl:161-192), taken from the top level of Φ̂
+int ma_surface_outputs(ssg_ctx* c) {
+    PathBuffers<double>& D = c->d;
+    const size_t lb = (size_t)c->ny * sizeof(cpx<double>);
+    HIP_OK(c, hipMemcpy2DAsync(D.Tp, lb, c->ma_phi + (size_t)(c->nz - 1) * c->ny, lb * c->nz, lb, c->ncols_full,
+                               hipMemcpyDeviceToDevice, c->stream));
+    const double scale = 1.0 / ((double)c->nx * c->ny);
+    ColsArgs<double> a{};
+    a.inA = D.Tp;
+    a.outA = D.Fx;
+    a.outB = D.Fy;
+    a.scale = scale;
+    int rc = launch_cols_args<double, OP_VEL_SPEC>(c, a, c->ncols_full, 1);
+    if (rc) return rc;
+    RowsInvArgs<double> w{};
+    w.P0 = D.Fx;
+    w.P1 = D.Fy;
+    w.ncols_in = c->ncols_full;
+    w.w_out = D.w;
+    if ((rc = launch_rows_inv<double, MODE_PLANES, EPI_STORE_W>(c, w))) return rc;
+    if ((rc = launch_cols<double, OP_INV>(c, D.Tp, nullptr, D.Tp, nullptr, c->ncols_full, scale))) return rc;
+    RowsInvArgs<double> f{};
+    f.P0 = D.Tp;
+    f.ncols_in = c->ncols_full;
+    f.out_r = c->phi;
+    if ((rc = launch_rows_inv<double, MODE_PAIR, EPI_STORE>(c, f))) return rc;
+    if (c->precision == SSG_FP32_MIXED) {
+        const long n = (long)c->nx * c->ny;
+        hipLaunchKernelGGL((k_w_convert<float>), dim3(grid1d(n)), dim3(256), 0, c->stream, D.w, c->f.w, n);
+        c->launches++;
+        HIP_OK(c, hipGetLastError());
+    }
+    return SSG_OK;
+}
+
+int ma_check_params(ssg_ctx* c, const ssg_ma_params* p) {
+    if (!p) return set_err(c, SSG_ERR_ARG, "null Monge-Ampere parameters");
+    if (!(p->eps >= 0) || !std::isfinite(p->eps)) return set_err(c, SSG_ERR_ARG, "eps must be finite and >= 0");
+    if (!(p->tol > 0)) return set_err(c, SSG_ERR_ARG, "tol must be > 0");
+    if (p->maxiter < 0) return set_err(c, SSG_ERR_ARG, "maxiter must be >= 0");
+    if (p->variant != SSG_MA_BODY && p->variant != SSG_MA_SURFACE)
+        return set_err(c, SSG_ERR_ARG, "unknown Monge-Ampere variant %d", p->variant);
+    return SSG_OK;
+}
+
+// full solve for the buoyancy `bs` (device, FP64)
+int ma_solve_dev(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_diag* d) {
+    int rc = ma_alloc(c);
+    if (rc) return rc;
+    if ((rc = rfft2_device(c, bs, c->ma_b2))) return rc;
+    if ((rc = ma_iterate(c, p, d))) return rc;
+    if ((rc = ma_surface_outputs(c))) return rc;
+    c->last_diag = *d;
     return SSG_OK;
 }
 
@@ -567,12 +890,14 @@ int finite_check(ssg_ctx* c, int64_t* nonfinite, double* sum, double* sum2) {
     return SSG_OK;
 }
 
-// rfft2 of a device FP64 field into c->d.Tp (this library's layout, FP64)
-int rfft2_device(ssg_ctx* c, const double* src) {
+// rfft2 of a device FP64 field into dst (default c->d.Tp), this library's
+// layout, FP64
+int rfft2_device(ssg_ctx* c, const double* src, cpx<double>* dst) {
     PathBuffers<double>& D = c->d;
-    int rc = launch_rows_fwd<double, double, SRC_PAIR>(c, src, nullptr, D.Tp, nullptr, c->ncols_full);
+    if (!dst) dst = D.Tp;
+    int rc = launch_rows_fwd<double, double, SRC_PAIR>(c, src, nullptr, dst, nullptr, c->ncols_full);
     if (rc) return rc;
-    return launch_cols<double, OP_FWD>(c, D.Tp, nullptr, D.Tp, nullptr, c->ncols_full, 1.0);
+    return launch_cols<double, OP_FWD>(c, dst, nullptr, dst, nullptr, c->ncols_full, 1.0);
 }
 
 double parseval_of_tp(ssg_ctx* c, int* rc_out) {
@@ -665,6 +990,7 @@ int ssg_destroy(ssg_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_path(c->d);
     free_path(c->f);
+    ma_free(c);
     (void)hipFree(c->b);
     (void)hipFree(c->phi);
     (void)hipFree(c->dscratch);
@@ -812,6 +1138,72 @@ int ssg_profile_read(ssg_ctx* c, int32_t kind, double* total_ms, int64_t* count,
     if (total_ms) *total_ms = ms;
     if (count) *count = n;
     if (bytes) *bytes = by;
+    return SSG_OK;
+}
+
+int ssg_ma_solve(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
+    if (!c) return SSG_ERR_ARG;
+    int rc = ma_check_params(c, p);
+    if (rc) return rc;
+    HIP_OK(c, hipSetDevice(c->device));
+    ssg_ma_diag local{};
+    if (!d) d = &local;
+    HIP_OK(c, hipEventRecord(c->ev0, c->stream));
+    if ((rc = ma_solve_dev(c, c->b, p, d))) return rc;
+    HIP_OK(c, hipEventRecord(c->ev1, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIP_OK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    d->solve_ms = ms;
+    c->last_diag = *d;
+    if (!d->converged)
+        return set_err(c, SSG_ERR_NOT_CONVERGED, "not converged after %d iterations (rel. residual %.3e)", d->iterations,
+                       d->relative_residual);
+    return SSG_OK;
+}
+
+int ssg_get_phi3d(ssg_ctx* c, double* host, int64_t n) {
+    if (!c) return SSG_ERR_ARG;
+    if (!host) return set_err(c, SSG_ERR_ARG, "null pointer");
+    const int64_t need = (int64_t)c->nx * c->ny * c->nz;
+    if (n != need) return set_err(c, SSG_ERR_ARG, "size %lld != nx*ny*nz = %lld", (long long)n, (long long)need);
+    if (!c->ma_ready) return set_err(c, SSG_ERR_ARG, "no Monge-Ampere solution yet");
+    HIP_OK(c, hipSetDevice(c->device));
+    const int lines = c->ncols_full * c->nz;
+    ColsArgs<double> a{};
+    a.inA = c->ma_phi;
+    a.outA = c->ma_p[0];
+    a.scale = 1.0 / ((double)c->nx * c->ny);
+    int rc = launch_cols_args<double, OP_INV>(c, a, lines, c->nz);
+    if (rc) return rc;
+    RowsInvArgs<double> r{};
+    r.P0 = c->ma_p[0];
+    r.ncols_in = c->ncols_full;
+    r.out_r = reinterpret_cast<double*>(c->ma_p[1]);
+    if ((rc = launch_rows_inv<double, MODE_PAIR, EPI_STORE>(c, r, c->nz * c->ny, &c->g3))) return rc;
+    HIP_OK(c, hipMemcpyAsync(host, c->ma_p[1], (size_t)need * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+int ssg_set_coupling(ssg_ctx* c, const ssg_ma_params* p) {
+    if (!c) return SSG_ERR_ARG;
+    if (!p) {
+        c->coupled = false;
+        return SSG_OK;
+    }
+    int rc = ma_check_params(c, p);
+    if (rc) return rc;
+    HIP_OK(c, hipSetDevice(c->device));
+    if ((rc = ma_alloc(c))) return rc;
+    c->ma_par = *p;
+    c->coupled = true;
+    return SSG_OK;
+}
+
+int ssg_last_ma_diag(ssg_ctx* c, ssg_ma_diag* d) {
+    if (!c || !d) return SSG_ERR_ARG;
+    *d = c->last_diag;
     return SSG_OK;
 }
 

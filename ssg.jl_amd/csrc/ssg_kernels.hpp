@@ -44,7 +44,10 @@ enum {
     OP_INV = 6,      // inverse y-FFT only
     OP_DX = 7,       // Dx P
     OP_DY = 8,       // Dy P
-    OP_MASK = 9      // M P
+    OP_MASK = 9,     // M P
+    OP_MA_DERIV = 10,  // Monge-Ampere: P -> (Pxx, Pyy, Pxy), P spectral
+    OP_MA_DIFF = 11,   // Monge-Ampere: y-FFT of D, residual sums against the previous D
+    OP_VEL_SPEC = 12   // U = -Dy P, V = Dx P with P already spectral in y
 };
 
 struct Grid {
@@ -90,6 +93,26 @@ __device__ __forceinline__ R sym_mask(const Grid& g, int c, int s) {
     if (c <= g.K - 1) return (R)1;
     return c == g.K ? (R)0.5 : (R)0;
 }
+// d2/dxdy through irfft2 (ddx then ddy, src/poisson.jl:190-193): -kx*ky;
+// zero on the y-Nyquist row except on the x-Nyquist column, where the
+// reference's kx = -nx/2 gives +(nx/2)|ky| on both ky half planes.
+template <typename R>
+__device__ __forceinline__ R sym_dxy(const Grid& g, int c, int s) {
+    if (2 * c < g.nx) return 2 * s == -g.ny ? (R)0 : (R)(-(c * g.dkx) * (s * g.dky));
+    return (R)((g.nx / 2) * g.dkx * (s < 0 ? -s : s) * g.dky);
+}
+// d2/dx2 and d2/dy2 through irfft2: -kx^2, -ky^2 (Nyquist entries included)
+template <typename R>
+__device__ __forceinline__ R sym_dxx(const Grid& g, int c) {
+    const double k = (2 * c < g.nx ? c : g.nx / 2) * g.dkx;
+    return (R)(-k * k);
+}
+template <typename R>
+__device__ __forceinline__ R sym_dyy(const Grid& g, int s) {
+    const double k = s * g.dky;
+    return (R)(-k * k);
+}
+
 // exponential_transfer (src/filter.jl:119-126) at k_norm = |k| / kmax
 template <typename R>
 __device__ __forceinline__ R sym_filter(const Grid& g, int c, int s, R strength, int order, R cutoff,
@@ -208,7 +231,36 @@ struct ColsArgs {
     int fdelta;            // filter returns (sigma - 1) * P
     const cpx<R>* twy;
     Grid g;
+    // Monge-Ampere extensions (lines are (kx, level) pairs, kx = line / nlev)
+    int nlev;
+    cpx<R>* outC;
+    const cpx<double>* B2;  // 2D forcing spectrum [kx][ky] (variant A body force)
+    double eps;
+    int body;               // 1: B2 is the interior right-hand side on every level
+    double* partial;        // per-block (sum |eps dD|^2, sum |rhs|^2) over interior levels
 };
+
+// per-block sums of W values into out[blockIdx.x * W + k]; blockDim is a
+// power of two <= 256 (tree reduction in LDS, no partial-wave shuffles)
+template <int W>
+__device__ __forceinline__ void block_sum_g(double (&v)[W], double* out) {
+    __shared__ double red_g[W][256];
+    const int tid = threadIdx.x, nt = blockDim.x;
+#pragma unroll
+    for (int k = 0; k < W; ++k) red_g[k][tid] = v[k];
+    __syncthreads();
+    for (int st = nt >> 1; st > 0; st >>= 1) {
+        if (tid < st) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) red_g[k][tid] += red_g[k][tid + st];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) out[(size_t)blockIdx.x * W + k] = red_g[k][0];
+    }
+}
 
 template <typename R, int M, int E, int OP>
 __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
@@ -219,40 +271,120 @@ __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
     const int l = threadIdx.x / T, t = threadIdx.x - l * T;
-    const int c = blockIdx.x * a.G + l;
-    const bool live = c < a.ncols;
-    const int pitch = a.g.pitch;
+    const int line = blockIdx.x * a.G + l;
+    const bool live = line < a.ncols;
+    const int pitch = a.g.ny;                     // lines are contiguous runs of ny
+    const int c = a.nlev > 1 ? line / a.nlev : line;  // kx column of this line
     C* buf = lds + l * LE;
     typename F::Twiddles tw;
     tw.load(t, a.twy);
     constexpr bool TWO_IN = (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC);
     C va[E];
     C vb[E];
-    const C* srcA = a.inA + (size_t)(live ? c : 0) * pitch;
+    const C* srcA = a.inA + (size_t)(live ? line : 0) * pitch;
 #pragma unroll
     for (int m = 0; m < E; ++m) va[m] = live ? srcA[t + T * m] : C{0, 0};
     if constexpr (TWO_IN) {
         // issue the second plane's loads before the first transform
-        const C* srcB = a.inB + (size_t)(live ? c : 0) * pitch;
+        const C* srcB = a.inB + (size_t)(live ? line : 0) * pitch;
 #pragma unroll
         for (int m = 0; m < E; ++m) vb[m] = live ? srcB[t + T * m] : C{0, 0};
     }
 
+    if constexpr (OP == OP_MA_DERIV) {
+        C vb2[E], vc[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int s = signed_k(t + T * m, a.g.ny);
+            const C p = cscale(va[m], a.scale);
+            va[m] = cscale(p, sym_dxx<R>(a.g, c));
+            vb2[m] = cscale(p, sym_dyy<R>(a.g, s));
+            vc[m] = cscale(p, sym_dxy<R>(a.g, c, s));
+        }
+        F::template run<+1>(va, buf, t, tw);
+        F::template run<+1>(vb2, buf, t, tw);
+        F::template run<+1>(vc, buf, t, tw);
+        if (live) {
+            C* dA = a.outA + (size_t)line * pitch;
+            C* dB = a.outB + (size_t)line * pitch;
+            C* dC = a.outC + (size_t)line * pitch;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                dA[t + T * m] = va[m];
+                dB[t + T * m] = vb2[m];
+                dC[t + T * m] = vc[m];
+            }
+        }
+        return;
+    }
+    if constexpr (OP == OP_VEL_SPEC) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int s = signed_k(t + T * m, a.g.ny);
+            const R dy = sym_dy<R>(a.g, s), dx = sym_dx<R>(a.g, c, s);
+            vb[m] = cmul_i(va[m], dx * a.scale);
+            va[m] = cmul_i(va[m], -dy * a.scale);
+        }
+        F::template run<+1>(va, buf, t, tw);
+        F::template run<+1>(vb, buf, t, tw);
+        if (live) {
+            C* dA = a.outA + (size_t)line * pitch;
+            C* dB = a.outB + (size_t)line * pitch;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                dA[t + T * m] = va[m];
+                dB[t + T * m] = vb[m];
+            }
+        }
+        return;
+    }
     if (OP == OP_INV) {
         F::template run<+1>(va, buf, t, tw);
 #pragma unroll
         for (int m = 0; m < E; ++m) va[m] = cscale(va[m], a.scale);
         if (live) {
-            C* dst = a.outA + (size_t)c * pitch;
+            C* dst = a.outA + (size_t)line * pitch;
 #pragma unroll
             for (int m = 0; m < E; ++m) dst[t + T * m] = va[m];
         }
         return;
     }
     F::template run<-1>(va, buf, t, tw);
+    if constexpr (OP == OP_MA_DIFF) {
+        // va = D-hat of this (kx, level) line; residual of the iterate that
+        // produced it is eps * (D_prev - D) on interior levels
+        const int lev = a.nlev > 1 ? line - c * a.nlev : 0;
+        const bool interior = live && lev >= 1 && lev <= a.nlev - 2;
+        const double wgt = (c == 0 || 2 * c == a.g.nx) ? 1.0 : 2.0;
+        double acc[2] = {0.0, 0.0};
+        C* dp = a.outA + (size_t)(live ? line : 0) * pitch;
+        const cpx<double>* b2 = a.B2 + (size_t)c * a.g.ny;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int y = t + T * m;
+            if (live) {
+                const C prev = dp[y];
+                if (interior) {
+                    const double rx = a.eps * ((double)prev.x - (double)va[m].x);
+                    const double ry = a.eps * ((double)prev.y - (double)va[m].y);
+                    double hx = a.eps * (double)va[m].x, hy = a.eps * (double)va[m].y;
+                    if (a.body) {
+                        const cpx<double> bb = b2[y];
+                        hx += bb.x;
+                        hy += bb.y;
+                    }
+                    acc[0] += wgt * (rx * rx + ry * ry);
+                    acc[1] += wgt * (hx * hx + hy * hy);
+                }
+                dp[y] = va[m];
+            }
+        }
+        block_sum_g<2>(acc, a.partial);
+        return;
+    }
     if (OP == OP_FWD) {
         if (live) {
-            C* dst = a.outA + (size_t)c * pitch;
+            C* dst = a.outA + (size_t)line * pitch;
 #pragma unroll
             for (int m = 0; m < E; ++m) dst[t + T * m] = va[m];
         }
@@ -286,7 +418,7 @@ __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
         }
         F::template run<+1>(vb, buf, t, tw);
         if (live) {
-            C* dst = a.outB + (size_t)c * pitch;
+            C* dst = a.outB + (size_t)line * pitch;
 #pragma unroll
             for (int m = 0; m < E; ++m) dst[t + T * m] = vb[m];
         }
@@ -309,7 +441,7 @@ __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
     }
     F::template run<+1>(va, buf, t, tw);
     if (live) {
-        C* dst = a.outA + (size_t)c * pitch;
+        C* dst = a.outA + (size_t)line * pitch;
 #pragma unroll
         for (int m = 0; m < E; ++m) dst[t + T * m] = va[m];
     }
@@ -335,6 +467,7 @@ struct RowsInvArgs {
     R* fold;               // AB2 previous tendency
     R* out_r;              // EPI_STORE
     cpx<R>* w_out;         // EPI_STORE_W
+    double* bstage;        // RK1/RK2: also store the stage buoyancy (coupled mode)
     double c1, c2;         // stage coefficients
     int first;             // AB2: forward Euler on step 0
     // next-stage flux emission
@@ -496,6 +629,10 @@ __global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
                 nbB = pbB[m] + (double)tB;
                 a.b[bA + x] = nbA;
                 a.b[bB + x] = nbB;
+            }
+            if ((EPI == EPI_RK1 || EPI == EPI_RK2) && a.bstage) {
+                a.bstage[bA + x] = nbA;
+                a.bstage[bB + x] = nbB;
             }
             sA[m] = (R)nbA;
             sB[m] = (R)nbB;

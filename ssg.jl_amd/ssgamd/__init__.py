@@ -49,8 +49,12 @@ EXPORTED_SYMBOLS = (
     "ssg_create", "ssg_destroy", "ssg_last_error", "ssg_sync", "ssg_set_field", "ssg_get_field",
     "ssg_set_field_device", "ssg_get_field_device", "ssg_timestep", "ssg_tendency", "ssg_jacobian",
     "ssg_rfft_2d", "ssg_irfft_2d", "ssg_apply_exponential_filter", "ssg_diagnostics", "ssg_last_timing",
-    "ssg_profile", "ssg_profile_read", "ssg_tuning",
+    "ssg_profile", "ssg_profile_read", "ssg_tuning", "ssg_ma_solve", "ssg_get_phi3d", "ssg_set_coupling",
+    "ssg_last_ma_diag",
 )
+
+MA_BODY = 0      # as wired: b on every level, surface slope 0
+MA_SURFACE = 1   # documented physics: surface slope = b
 
 
 class TimeScheme(enum.IntEnum):
@@ -93,6 +97,23 @@ class _Diag(ctypes.Structure):
                 ("ke_surface", ctypes.c_double), ("nonfinite", ctypes.c_int64)]
 
 
+class _MaParams(ctypes.Structure):
+    _fields_ = [("eps", ctypes.c_double), ("tol", ctypes.c_double), ("maxiter", ctypes.c_int32),
+                ("variant", ctypes.c_int32), ("warm", ctypes.c_int32)]
+
+
+class _MaDiag(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int32), ("iterations", ctypes.c_int32), ("final_residual", ctypes.c_double),
+                ("relative_residual", ctypes.c_double), ("history", ctypes.c_double * 64),
+                ("solve_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        n = min(self.iterations + 1, 64)
+        return {"converged": bool(self.converged), "iterations": self.iterations,
+                "final_residual": self.final_residual, "relative_residual": self.relative_residual,
+                "convergence_history": list(self.history[:n]), "solve_time": self.solve_ms / 1e3}
+
+
 _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -124,6 +145,10 @@ def load_library(path: str | None = None):
     lib.ssg_last_timing.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int64)]
     lib.ssg_profile.argtypes = [vp, ctypes.c_int32]
     lib.ssg_tuning.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _dp]
+    lib.ssg_ma_solve.argtypes = [vp, ctypes.POINTER(_MaParams), ctypes.POINTER(_MaDiag)]
+    lib.ssg_get_phi3d.argtypes = [vp, _dp, ctypes.c_int64]
+    lib.ssg_set_coupling.argtypes = [vp, ctypes.POINTER(_MaParams)]
+    lib.ssg_last_ma_diag.argtypes = [vp, ctypes.POINTER(_MaDiag)]
     lib.ssg_profile_read.argtypes = [vp, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), _dp]
     for n in EXPORTED_SYMBOLS:
         getattr(lib, n).restype = ctypes.c_char_p if n == "ssg_last_error" else ctypes.c_int
@@ -370,3 +395,43 @@ def diagnostics(domain: Domain) -> dict:
     domain.check(domain.lib.ssg_diagnostics(domain.handle, ctypes.byref(d)))
     return {"sum_b": d.sum_b, "sum_b2": d.sum_b2, "parseval_b": d.parseval_b, "ke_surface": d.ke_surface,
             "nonfinite": d.nonfinite}
+
+
+def _ma_params(eps, tol, maxiter, variant, warm):
+    return _MaParams(float(eps), float(tol), int(maxiter), int(variant), int(bool(warm)))
+
+
+def solve_monge_ampere(fields: Fields, domain: Domain, *, eps: float = 0.1, tol: float = 1e-10, maxiter: int = 50,
+                       variant: int = MA_BODY, warm: bool = False) -> dict:
+    """``solve_monge_ampere_fields!`` (src/poisson.jl:1068-1090) on the GPU.
+
+    Solves the reference residual operator for the current ``bs`` and sets
+    ``phis`` (and the surface velocities) from the top level.  Returns the
+    reference's diagnostics tuple as a dict; ``converged`` is False (no
+    exception) when ``maxiter`` is reached, as in ``solve_ssg_equation``."""
+    p = _ma_params(eps, tol, maxiter, variant, warm)
+    d = _MaDiag()
+    rc = domain.lib.ssg_ma_solve(domain.handle, ctypes.byref(p), ctypes.byref(d))
+    if rc not in (SSG_OK, SSG_ERR_NOT_CONVERGED):
+        domain.check(rc)
+    out = d.as_dict()
+    out["epsilon_parameter"] = float(eps)
+    return out
+
+
+def get_phi3d(domain: Domain) -> np.ndarray:
+    """``fields.φ`` after a Monge-Ampère solve, shape (nz, ny, nx)."""
+    out = np.empty((domain.Nz, domain.Ny, domain.Nx))
+    domain.check(domain.lib.ssg_get_phi3d(domain.handle, out.ctypes.data_as(_dp), out.size))
+    return out
+
+
+def set_coupling(domain: Domain, *, eps: float | None = 0.1, tol: float = 1e-10, maxiter: int = 50,
+                 variant: int = MA_BODY, warm: bool = True) -> None:
+    """Coupled stepping: re-solve the Monge-Ampère problem at every RK stage
+    (``eps=None`` restores the reference's frozen φₛ)."""
+    if eps is None:
+        domain.check(domain.lib.ssg_set_coupling(domain.handle, None))
+        return
+    p = _ma_params(eps, tol, maxiter, variant, warm)
+    domain.check(domain.lib.ssg_set_coupling(domain.handle, ctypes.byref(p)))
