@@ -103,6 +103,21 @@ __host__ __device__ constexpr int tw_offset(int n, int e, int ns) {
     return tw_count_from(n, e, e) - tw_count_from(n, e, ns);
 }
 
+// Synchronisation between the stages of one line's transform.  When a line
+// is owned by threads of a single wavefront (T <= 64, T a power of two, lines
+// laid out contiguously) the exchange stays inside the wave: LDS operations
+// of a wave complete in order, so a wave-scope fence and a scheduling
+// barrier replace the block barrier and lines of other waves run
+// uncoupled.  Cooperative phases across lines keep __syncthreads().
+template <int T>
+__device__ __forceinline__ void line_sync() {
+    if constexpr (T <= 64) {
+        __syncwarp();
+    } else {
+        __syncthreads();
+    }
+}
+
 // Stockham line FFT.  v[] holds x[t + T*m] on entry and X[t + T*m] on exit.
 // `buf` is this line's LDS buffer (lds_line_elems(N) entries).  The twiddle
 // factors a thread needs depend only on t, so they are fetched once per
@@ -145,14 +160,14 @@ template <typename R, int N, int E> struct LineFFT {
         constexpr bool LAST = (NS * RAD == N);
         constexpr int OFF = tw_offset(N, E, NS);
         C u[E];
-        __syncthreads();
+        line_sync<T>();
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int j = t + T * q;
 #pragma unroll
             for (int r = 0; r < RAD; ++r) u[q * RAD + r] = buf[lpad(j + r * (N / RAD))];
         }
-        __syncthreads();
+        line_sync<T>();
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int j = t + T * q;
@@ -177,12 +192,85 @@ template <typename R, int N, int E> struct LineFFT {
         if constexpr (!LAST) stage<DIR, NS * RAD>(v, buf, t, tw);
     }
 
+    // NB independent lines of the same thread group transformed in lockstep:
+    // one barrier pair per stage for all of them, NB-fold independent work
+    // between barriers.
+    template <int DIR, int NS, int NB>
+    __device__ __forceinline__ static void stage_n(C (&v)[NB][E], C* const (&buf)[NB], int t, const Twiddles& tw) {
+        constexpr int RAD = cmin(E, N / NS);
+        constexpr int Q = E / RAD;
+        constexpr bool LAST = (NS * RAD == N);
+        constexpr int OFF = tw_offset(N, E, NS);
+        C u[NB][E];
+        line_sync<T>();
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int j = t + T * q;
+#pragma unroll
+                for (int r = 0; r < RAD; ++r) u[b][q * RAD + r] = buf[b][lpad(j + r * (N / RAD))];
+            }
+        line_sync<T>();
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int j = t + T * q;
+            const int k = j % NS;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                C* w = u[b] + q * RAD;
+#pragma unroll
+                for (int r = 1; r < RAD; ++r) {
+                    C f = tw.w[OFF + q * (RAD - 1) + r - 1];
+                    if (DIR > 0) f.y = -f.y;
+                    w[r] = cmul(w[r], f);
+                }
+                SmallDFT<R, RAD, DIR>::run(w);
+                if (LAST) {
+#pragma unroll
+                    for (int r = 0; r < RAD; ++r) v[b][q + r * Q] = w[r];
+                } else {
+                    const int base = (j / NS) * NS * RAD + k;
+#pragma unroll
+                    for (int r = 0; r < RAD; ++r) buf[b][lpad(base + r * NS)] = w[r];
+                }
+            }
+        }
+        if constexpr (!LAST) stage_n<DIR, NS * RAD, NB>(v, buf, t, tw);
+    }
+
+    template <int DIR, int NB>
+    __device__ __forceinline__ static void run_n(C (&v)[NB][E], C* const (&buf)[NB], int t, const Twiddles& tw) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) SmallDFT<R, E, DIR>::run(v[b]);
+        if constexpr (T > 1) {
+            line_sync<T>();
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 0; r < E; ++r) buf[b][lpad(t * E + r)] = v[b][r];
+            stage_n<DIR, E, NB>(v, buf, t, tw);
+        }
+    }
+
+    // NB lines whose inputs sit in LDS in natural order
+    template <int DIR, int NB>
+    __device__ __forceinline__ static void run_n_from_lds(C (&v)[NB][E], C* const (&buf)[NB], int t,
+                                                          const Twiddles& tw) {
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[b][m] = buf[b][lpad(t + T * m)];
+        run_n<DIR, NB>(v, buf, t, tw);
+    }
+
     // Transform from registers.
     template <int DIR>
     __device__ __forceinline__ static void run(C* v, C* buf, int t, const Twiddles& tw) {
         SmallDFT<R, E, DIR>::run(v);
         if constexpr (T > 1) {
-            __syncthreads();  // earlier readers of buf are done
+            line_sync<T>();  // earlier readers of buf are done
 #pragma unroll
             for (int r = 0; r < E; ++r) buf[lpad(t * E + r)] = v[r];
             stage<DIR, E>(v, buf, t, tw);

@@ -8,7 +8,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -79,6 +82,11 @@ struct ssg_ctx {
     ssg_ma_params ma_par{};
     double* bst = nullptr;            // stage buoyancy (coupled mode)
     ssg_ma_diag last_diag{};
+    struct GraphCache {
+        hipGraphExec_t exec = nullptr;
+        int len = 0;
+        ssg_time_params p{};
+    } gcache[2];
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int64_t last_launches = 0;
@@ -169,9 +177,9 @@ int with_size_ept(int n, int e, Fn&& fn) {
 // lines per block: at most 256 threads and kMaxLdsBytes of LDS, a power of
 // two that divides `rows`
 template <typename R>
-int lines_per_block(int n, int e, int rows) {
+int lines_per_block(int n, int e, int rows, int nbuf = 1) {
     const int T = n / e;
-    const int le = lds_line_elems(n) * (int)sizeof(cpx<R>);
+    const int le = lds_line_elems(n) * (int)sizeof(cpx<R>) * nbuf;
     int G = 1;
     while (2 * G * T <= 256 && 2 * G * le <= kMaxLdsBytes && rows % (2 * G) == 0) G *= 2;
     return G;
@@ -232,7 +240,7 @@ int launch_cols_args(ssg_ctx* c, ColsArgs<R> a, int nlines, int nlev) {
         constexpr int M = decltype(tag)::value;
         constexpr int E = decltype(etag)::value;
         constexpr int T = M / E;
-        const int le = lds_line_elems(M) * (int)sizeof(cpx<R>);
+        const int le = lds_line_elems(M) * (int)sizeof(cpx<R>) * cols_nbuf(OP);
         int G = 1;
         while (2 * G * T <= 256 && 2 * G * le <= kMaxLdsBytes && 2 * G <= nlines) G *= 2;
         const int nblk = (nlines + G - 1) / G;
@@ -285,13 +293,13 @@ int launch_rows_inv(ssg_ctx* c, RowsInvArgs<R> a, int rows = 0, const Grid* grid
         constexpr int E = decltype(etag)::value;
         constexpr int T = N / E;
         const int rows_per_line = MODE == MODE_PAIR ? 2 : 1;
-        const int G = lines_per_block<R>(N, E, rows / rows_per_line);
+        const int G = lines_per_block<R>(N, E, rows / rows_per_line, 2);
         const int nblk = rows / (rows_per_line * G);
         a.G = G;
         a.nblk = nblk;
         a.twx = B.twx;
         a.g = grid ? *grid : c->g;
-        const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<R>);
+        const size_t lds = (size_t)G * 2 * lds_line_elems(N) * sizeof(cpx<R>);
         const double n2 = (double)c->nx * rows, sc = sizeof(cpx<R>);
         double bytes = (MODE == MODE_PAIR ? 1.0 : 2.0) * a.ncols_in * rows * sc;
         if (EPI == EPI_STORE) bytes += n2 * sizeof(R);
@@ -445,11 +453,34 @@ int filter_b(ssg_ctx* c, double strength, int order, double cutoff, int emit) {
 template <typename R>
 int run_kind(ssg_ctx* c, int kind);
 
+// Tuning results are shared by every context of the process with the same
+// grid, path and device, so that contexts agree on launch shapes and
+// therefore on roundoff (bitwise-reproducible results within a process;
+// SSG_EPT pins the shapes across processes).
+struct TuneEntry {
+    int ept[3];
+    double us[3][3];
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, TuneEntry> g_tune_cache;
+
 template <typename R>
 int autotune(ssg_ctx* c) {
     const int p = std::is_same<R, float>::value ? 1 : 0;
     if (c->tuned[p]) return SSG_OK;
     c->tuned[p] = true;
+    const auto key = std::make_tuple(c->nx, c->ny, p, c->device);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end() && !getenv("SSG_EPT")) {
+            for (int k = 0; k < 3; ++k) {
+                c->ept[p][k] = it->second.ept[k];
+                for (int i = 0; i < 3; ++i) c->tune_us[p][k][i] = it->second.us[k][i];
+            }
+            return SSG_OK;
+        }
+    }
     if (const char* env = getenv("SSG_EPT")) {
         int e0 = 0, e1 = 0, e2 = 0;
         if (sscanf(env, "%d,%d,%d", &e0, &e1, &e2) == 3) {
@@ -497,6 +528,15 @@ int autotune(ssg_ctx* c) {
     }
     c->prof = prof_saved;
     c->launches = launches_saved;
+    if (rc == SSG_OK) {
+        TuneEntry e{};
+        for (int k = 0; k < 3; ++k) {
+            e.ept[k] = c->ept[p][k];
+            for (int i = 0; i < 3; ++i) e.us[k][i] = c->tune_us[p][k][i];
+        }
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache.emplace(key, e);
+    }
     return rc;
 }
 
@@ -543,39 +583,110 @@ int run_steps_coupled(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, i
     return SSG_OK;
 }
 
+// Kernels of one timestep! (src/timestep.jl:267-285) starting from step
+// number `step_before`; `last` suppresses the next-stage flux transform that
+// nothing would read.
+template <typename R>
+int issue_step(ssg_ctx* c, const ssg_time_params* p, int64_t step_before, bool last) {
+    int rc;
+    const double dt = p->dt;
+    const int64_t step_after = step_before + 1;
+    const bool filt = p->filter_freq > 0 && (step_after % p->filter_freq) == 0;
+    const int emit_final = (!filt && !last) ? 1 : 0;
+    if (p->scheme == SSG_RK3) {
+        // src/timestep.jl:195-242
+        if ((rc = tendency_cols<R>(c, 0))) return rc;
+        if ((rc = stage_inverse<R, EPI_RK1>(c, dt / 3.0, 1, 0))) return rc;
+        if ((rc = tendency_cols<R>(c, 0))) return rc;
+        if ((rc = stage_inverse<R, EPI_RK2>(c, 2.0 * dt / 3.0, 1, 0))) return rc;
+        if ((rc = tendency_cols<R>(c, 0))) return rc;
+        if ((rc = stage_inverse<R, EPI_RK3>(c, dt / 4.0, emit_final, 0))) return rc;
+    } else {
+        // src/timestep.jl:152-190
+        if ((rc = tendency_cols<R>(c, p->ab2_reference_aliasing))) return rc;
+        const int first = (step_before == 0) ? 1 : 0;
+        if ((rc = stage_inverse<R, EPI_AB2>(c, dt, emit_final, first))) return rc;
+    }
+    if (filt) {
+        // src/timestep.jl:280-281
+        if ((rc = filter_b<R>(c, p->filter_strength, p->filter_order, p->filter_cutoff, last ? 0 : 1))) return rc;
+    }
+    return SSG_OK;
+}
+
+bool same_params(const ssg_time_params& a, const ssg_time_params& b) {
+    return a.dt == b.dt && a.scheme == b.scheme && a.filter_freq == b.filter_freq &&
+           a.filter_strength == b.filter_strength && a.filter_order == b.filter_order &&
+           a.filter_cutoff == b.filter_cutoff && a.ab2_reference_aliasing == b.ab2_reference_aliasing;
+}
+
+// A hipGraph of one filter cycle (filter_freq steps, the filter on the last;
+// 16 steps when the filter is off), captured on first use and replayed for
+// every aligned cycle.  Kernel arguments are fixed by the parameters, so the
+// graph is rebuilt when they change.
+template <typename R>
+int cycle_graph(ssg_ctx* c, const ssg_time_params* p, int len, int64_t step0, hipGraphExec_t* out) {
+    const int path = std::is_same<R, float>::value ? 1 : 0;
+    auto& gc = c->gcache[path];
+    if (gc.exec && gc.len == len && same_params(gc.p, *p)) {
+        *out = gc.exec;
+        return SSG_OK;
+    }
+    if (gc.exec) {
+        (void)hipGraphExecDestroy(gc.exec);
+        gc.exec = nullptr;
+    }
+    HIP_OK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = SSG_OK;
+    for (int i = 0; i < len && rc == SSG_OK; ++i) rc = issue_step<R>(c, p, step0 + i, false);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return set_err(c, SSG_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return set_err(c, SSG_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    gc.exec = ex;
+    gc.len = len;
+    gc.p = *p;
+    *out = ex;
+    return SSG_OK;
+}
+
 template <typename R>
 int run_steps(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, int64_t nsteps) {
     if (c->coupled) return run_steps_coupled<R>(c, p, s, nsteps);
     int rc = flux_forward<R>(c, c->ncols_tend);
     if (rc) return rc;
-    const double dt = p->dt;
-    for (int64_t it = 0; it < nsteps; ++it) {
-        const int64_t step_after = s->step + 1;
-        const bool filt = p->filter_freq > 0 && (step_after % p->filter_freq) == 0;
-        const bool last = (it == nsteps - 1);
-        const int emit_final = (!filt && !last) ? 1 : 0;
-        if (p->scheme == SSG_RK3) {
-            // src/timestep.jl:195-242
-            if ((rc = tendency_cols<R>(c, 0))) return rc;
-            if ((rc = stage_inverse<R, EPI_RK1>(c, dt / 3.0, 1, 0))) return rc;
-            if ((rc = tendency_cols<R>(c, 0))) return rc;
-            if ((rc = stage_inverse<R, EPI_RK2>(c, 2.0 * dt / 3.0, 1, 0))) return rc;
-            if ((rc = tendency_cols<R>(c, 0))) return rc;
-            if ((rc = stage_inverse<R, EPI_RK3>(c, dt / 4.0, emit_final, 0))) return rc;
+    const char* genv = getenv("SSG_GRAPH");
+    const bool graphs = !(genv && genv[0] == '0') && !c->prof;
+    const int F = p->filter_freq > 0 ? p->filter_freq : 16;
+    int64_t it = 0;
+    while (it < nsteps) {
+        const int64_t remaining = nsteps - it;
+        const bool aligned = p->filter_freq <= 0 || (s->step % F) == 0;
+        const bool ab2_first = p->scheme == SSG_AB2_LOWSTORAGE && s->step == 0;
+        int adv = 1;
+        if (graphs && aligned && !ab2_first && remaining > F) {
+            hipGraphExec_t ex = nullptr;
+            if ((rc = cycle_graph<R>(c, p, F, s->step, &ex))) return rc;
+            HIP_OK(c, hipGraphLaunch(ex, c->stream));
+            c->launches += (p->scheme == SSG_RK3 ? 6 : 2) * F + (p->filter_freq > 0 ? 3 : 0);
+            adv = F;
         } else {
-            // src/timestep.jl:152-190
-            if ((rc = tendency_cols<R>(c, p->ab2_reference_aliasing))) return rc;
-            const int first = (s->step == 0) ? 1 : 0;
-            if ((rc = stage_inverse<R, EPI_AB2>(c, dt, emit_final, first))) return rc;
-            c->ab2_have_history = true;
+            if ((rc = issue_step<R>(c, p, s->step, it == nsteps - 1))) return rc;
         }
-        s->t += dt;
-        s->step = step_after;
-        s->dt_actual = dt;
-        if (filt) {
-            // src/timestep.jl:280-281
-            if ((rc = filter_b<R>(c, p->filter_strength, p->filter_order, p->filter_cutoff, last ? 0 : 1))) return rc;
+        for (int i = 0; i < adv; ++i) {
+            s->t += p->dt;
+            s->step += 1;
         }
+        s->dt_actual = p->dt;
+        if (p->scheme == SSG_AB2_LOWSTORAGE) c->ab2_have_history = true;
+        it += adv;
     }
     return SSG_OK;
 }
@@ -679,10 +790,10 @@ int ma_rows(ssg_ctx* c) {
         constexpr int N = decltype(tag)::value;
         constexpr int E = decltype(etag)::value;
         constexpr int T = N / E;
-        const int G = lines_per_block<double>(N, E, rows / 2);
+        const int G = lines_per_block<double>(N, E, rows / 2, 3);
         const int nblk = rows / (2 * G);
         MaRowsArgs<double> a{c->ma_p[0], c->ma_p[1], c->ma_p[2], c->ma_p[0], c->ncols_full, G, nblk, D.twx, c->g3};
-        const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<double>);
+        const size_t lds = (size_t)G * 3 * lds_line_elems(N) * sizeof(cpx<double>);
         const double bytes = 4.0 * c->ncols_full * rows * sizeof(cpx<double>);
         int slot, rc;
         if ((rc = prof_begin(c, &slot))) return rc;
@@ -1000,6 +1111,8 @@ int ssg_destroy(ssg_ctx* c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    for (auto& g : c->gcache)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SSG_OK;

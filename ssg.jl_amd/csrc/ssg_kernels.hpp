@@ -171,6 +171,23 @@ __device__ __forceinline__ void store_split(cpx<R>* lds, int G, int ncols, cpx<R
     }
 }
 
+// store_split over buffers at a line stride (buffer of line l at l*lstride)
+template <typename R, int N, int E>
+__device__ __forceinline__ void store_split_stride(cpx<R>* lds, int G, int lstride, int ncols, cpx<R>* PA,
+                                                   cpx<R>* PB, int pitch, int rowA0, int rowB0) {
+    using C = cpx<R>;
+    constexpr int LE = lds_line_elems(N);
+    const int total = ncols * G;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int kx = idx / G, l = idx - kx * G;
+        const C* buf = lds + l * lstride * LE;
+        const C z1 = buf[lpad(kx)];
+        const C z2 = cconj(buf[lpad((N - kx) & (N - 1))]);
+        PA[(size_t)kx * pitch + rowA0 + l] = C{(z1.x + z2.x) * (R)0.5, (z1.y + z2.y) * (R)0.5};
+        PB[(size_t)kx * pitch + rowB0 + l] = C{(z1.y - z2.y) * (R)0.5, -(z1.x - z2.x) * (R)0.5};
+    }
+}
+
 template <typename R, typename S, int N, int E, int SRC>
 __global__ __launch_bounds__(256) void k_rows_fwd(RowsFwdArgs<R, S> a) {
     using C = cpx<R>;
@@ -262,188 +279,161 @@ __device__ __forceinline__ void block_sum_g(double (&v)[W], double* out) {
     }
 }
 
+// LDS line buffers per column line for each operation
+__host__ __device__ constexpr int cols_nbuf(int op) {
+    return op == OP_MA_DERIV ? 3
+                             : ((op == OP_TEND || op == OP_TEND_Q || op == OP_JAC || op == OP_VEL || op == OP_VEL_SPEC)
+                                    ? 2
+                                    : 1);
+}
+
 template <typename R, int M, int E, int OP>
 __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
     using C = cpx<R>;
     using F = LineFFT<R, M, E>;
     constexpr int T = F::T;
     constexpr int LE = lds_line_elems(M);
+    constexpr int NBUF = cols_nbuf(OP);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
     const int l = threadIdx.x / T, t = threadIdx.x - l * T;
     const int line = blockIdx.x * a.G + l;
     const bool live = line < a.ncols;
-    const int pitch = a.g.ny;                     // lines are contiguous runs of ny
+    const int pitch = a.g.ny;                         // lines are contiguous runs of ny
     const int c = a.nlev > 1 ? line / a.nlev : line;  // kx column of this line
-    C* buf = lds + l * LE;
+    C* const buf1[1] = {lds + (l * NBUF) * LE};
     typename F::Twiddles tw;
     tw.load(t, a.twy);
     constexpr bool TWO_IN = (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC);
-    C va[E];
-    C vb[E];
+    C vv[2][E];
+    C (&va)[E] = vv[0];
+    C (&vb)[E] = vv[1];
     const C* srcA = a.inA + (size_t)(live ? line : 0) * pitch;
 #pragma unroll
     for (int m = 0; m < E; ++m) va[m] = live ? srcA[t + T * m] : C{0, 0};
     if constexpr (TWO_IN) {
-        // issue the second plane's loads before the first transform
         const C* srcB = a.inB + (size_t)(live ? line : 0) * pitch;
 #pragma unroll
         for (int m = 0; m < E; ++m) vb[m] = live ? srcB[t + T * m] : C{0, 0};
     }
+    auto store = [&](C* dst, const C (&v)[E]) {
+        if (live) {
+            C* d = dst + (size_t)line * pitch;
+#pragma unroll
+            for (int m = 0; m < E; ++m) d[t + T * m] = v[m];
+        }
+    };
 
     if constexpr (OP == OP_MA_DERIV) {
-        C vb2[E], vc[E];
+        C w3[3][E];
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int s = signed_k(t + T * m, a.g.ny);
             const C p = cscale(va[m], a.scale);
-            va[m] = cscale(p, sym_dxx<R>(a.g, c));
-            vb2[m] = cscale(p, sym_dyy<R>(a.g, s));
-            vc[m] = cscale(p, sym_dxy<R>(a.g, c, s));
+            w3[0][m] = cscale(p, sym_dxx<R>(a.g, c));
+            w3[1][m] = cscale(p, sym_dyy<R>(a.g, s));
+            w3[2][m] = cscale(p, sym_dxy<R>(a.g, c, s));
         }
-        F::template run<+1>(va, buf, t, tw);
-        F::template run<+1>(vb2, buf, t, tw);
-        F::template run<+1>(vc, buf, t, tw);
-        if (live) {
-            C* dA = a.outA + (size_t)line * pitch;
-            C* dB = a.outB + (size_t)line * pitch;
-            C* dC = a.outC + (size_t)line * pitch;
-#pragma unroll
-            for (int m = 0; m < E; ++m) {
-                dA[t + T * m] = va[m];
-                dB[t + T * m] = vb2[m];
-                dC[t + T * m] = vc[m];
-            }
-        }
+        C* const b3[3] = {lds + (l * NBUF) * LE, lds + (l * NBUF + 1) * LE, lds + (l * NBUF + 2) * LE};
+        F::template run_n<+1, 3>(w3, b3, t, tw);
+        store(a.outA, w3[0]);
+        store(a.outB, w3[1]);
+        store(a.outC, w3[2]);
         return;
-    }
-    if constexpr (OP == OP_VEL_SPEC) {
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int s = signed_k(t + T * m, a.g.ny);
-            const R dy = sym_dy<R>(a.g, s), dx = sym_dx<R>(a.g, c, s);
-            vb[m] = cmul_i(va[m], dx * a.scale);
-            va[m] = cmul_i(va[m], -dy * a.scale);
-        }
-        F::template run<+1>(va, buf, t, tw);
-        F::template run<+1>(vb, buf, t, tw);
-        if (live) {
-            C* dA = a.outA + (size_t)line * pitch;
-            C* dB = a.outB + (size_t)line * pitch;
-#pragma unroll
-            for (int m = 0; m < E; ++m) {
-                dA[t + T * m] = va[m];
-                dB[t + T * m] = vb[m];
-            }
-        }
-        return;
-    }
-    if (OP == OP_INV) {
-        F::template run<+1>(va, buf, t, tw);
+    } else if constexpr (OP == OP_INV) {
+        F::template run<+1>(va, buf1[0], t, tw);
 #pragma unroll
         for (int m = 0; m < E; ++m) va[m] = cscale(va[m], a.scale);
-        if (live) {
-            C* dst = a.outA + (size_t)line * pitch;
-#pragma unroll
-            for (int m = 0; m < E; ++m) dst[t + T * m] = va[m];
-        }
+        store(a.outA, va);
         return;
-    }
-    F::template run<-1>(va, buf, t, tw);
-    if constexpr (OP == OP_MA_DIFF) {
-        // va = D-hat of this (kx, level) line; residual of the iterate that
-        // produced it is eps * (D_prev - D) on interior levels
-        const int lev = a.nlev > 1 ? line - c * a.nlev : 0;
-        const bool interior = live && lev >= 1 && lev <= a.nlev - 2;
-        const double wgt = (c == 0 || 2 * c == a.g.nx) ? 1.0 : 2.0;
-        double acc[2] = {0.0, 0.0};
-        C* dp = a.outA + (size_t)(live ? line : 0) * pitch;
-        const cpx<double>* b2 = a.B2 + (size_t)c * a.g.ny;
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int y = t + T * m;
-            if (live) {
-                const C prev = dp[y];
-                if (interior) {
-                    const double rx = a.eps * ((double)prev.x - (double)va[m].x);
-                    const double ry = a.eps * ((double)prev.y - (double)va[m].y);
-                    double hx = a.eps * (double)va[m].x, hy = a.eps * (double)va[m].y;
-                    if (a.body) {
-                        const cpx<double> bb = b2[y];
-                        hx += bb.x;
-                        hy += bb.y;
-                    }
-                    acc[0] += wgt * (rx * rx + ry * ry);
-                    acc[1] += wgt * (hx * hx + hy * hy);
-                }
-                dp[y] = va[m];
-            }
-        }
-        block_sum_g<2>(acc, a.partial);
-        return;
-    }
-    if (OP == OP_FWD) {
-        if (live) {
-            C* dst = a.outA + (size_t)line * pitch;
-#pragma unroll
-            for (int m = 0; m < E; ++m) dst[t + T * m] = va[m];
-        }
-        return;
-    }
-    if constexpr (TWO_IN) {
-        F::template run<-1>(vb, buf, t, tw);
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int s = signed_k(t + T * m, a.g.ny);
-            const R dy = sym_dy<R>(a.g, s);
-            C r;
-            if (OP == OP_TEND_Q) {
-                r = cmul_i(vb[m], dy * (R)2);
+    } else {
+        C* const b2[2] = {lds + (l * NBUF) * LE, lds + (l * NBUF + (NBUF > 1 ? 1 : 0)) * LE};
+        // forward y-transform(s)
+        if constexpr (OP != OP_VEL_SPEC) {
+            if constexpr (TWO_IN) {
+                F::template run_n<-1, 2>(vv, b2, t, tw);
             } else {
-                const R dx = sym_dx<R>(a.g, c, s);
-                // i*dx*A + i*dy*B
-                r = C{-(dx * va[m].y + dy * vb[m].y), dx * va[m].x + dy * vb[m].x};
+                F::template run<-1>(va, buf1[0], t, tw);
             }
-            R f = -a.scale;
-            if (OP != OP_JAC) f *= sym_mask<R>(a.g, c, s);
-            va[m] = cscale(r, f);
         }
-    } else if (OP == OP_VEL) {
+        if constexpr (OP == OP_FWD) {
+            store(a.outA, va);
+            return;
+        } else if constexpr (OP == OP_MA_DIFF) {
+            // va = D-hat of this (kx, level) line; the residual of the iterate
+            // that produced it is eps * (D_prev - D) on interior levels
+            const int lev = a.nlev > 1 ? line - c * a.nlev : 0;
+            const bool interior = live && lev >= 1 && lev <= a.nlev - 2;
+            const double wgt = (c == 0 || 2 * c == a.g.nx) ? 1.0 : 2.0;
+            double acc[2] = {0.0, 0.0};
+            C* dp = a.outA + (size_t)(live ? line : 0) * pitch;
+            const cpx<double>* b2p = a.B2 + (size_t)c * a.g.ny;
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int s = signed_k(t + T * m, a.g.ny);
-            const R dy = sym_dy<R>(a.g, s), dx = sym_dx<R>(a.g, c, s);
-            vb[m] = cmul_i(va[m], dx * a.scale);    // V = Dx P
-            va[m] = cmul_i(va[m], -dy * a.scale);   // U = -Dy P
+            for (int m = 0; m < E; ++m) {
+                const int y = t + T * m;
+                if (live) {
+                    const C prev = dp[y];
+                    if (interior) {
+                        const double rx = a.eps * ((double)prev.x - (double)va[m].x);
+                        const double ry = a.eps * ((double)prev.y - (double)va[m].y);
+                        double hx = a.eps * (double)va[m].x, hy = a.eps * (double)va[m].y;
+                        if (a.body) {
+                            const cpx<double> bb = b2p[y];
+                            hx += bb.x;
+                            hy += bb.y;
+                        }
+                        acc[0] += wgt * (rx * rx + ry * ry);
+                        acc[1] += wgt * (hx * hx + hy * hy);
+                    }
+                    dp[y] = va[m];
+                }
+            }
+            block_sum_g<2>(acc, a.partial);
+            return;
+        } else if constexpr (OP == OP_VEL || OP == OP_VEL_SPEC) {
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const int s = signed_k(t + T * m, a.g.ny);
+                const R dy = sym_dy<R>(a.g, s), dx = sym_dx<R>(a.g, c, s);
+                vb[m] = cmul_i(va[m], dx * a.scale);   // V = Dx P
+                va[m] = cmul_i(va[m], -dy * a.scale);  // U = -Dy P
+            }
+            F::template run_n<+1, 2>(vv, b2, t, tw);
+            store(a.outA, va);
+            store(a.outB, vb);
+            return;
+        } else {
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const int s = signed_k(t + T * m, a.g.ny);
+                if constexpr (TWO_IN) {
+                    const R dy = sym_dy<R>(a.g, s);
+                    C r;
+                    if (OP == OP_TEND_Q) {
+                        r = cmul_i(vb[m], dy * (R)2);
+                    } else {
+                        const R dx = sym_dx<R>(a.g, c, s);
+                        // i*dx*A + i*dy*B
+                        r = C{-(dx * va[m].y + dy * vb[m].y), dx * va[m].x + dy * vb[m].x};
+                    }
+                    R f = -a.scale;
+                    if (OP != OP_JAC) f *= sym_mask<R>(a.g, c, s);
+                    va[m] = cscale(r, f);
+                } else if constexpr (OP == OP_FILT) {
+                    R sg = sym_filter<R>(a.g, c, s, a.fstrength, a.forder, a.fcutoff, a.inv_kmax);
+                    if (a.fdelta) sg -= (R)1;
+                    va[m] = cscale(va[m], sg * a.scale);
+                } else if constexpr (OP == OP_DX) {
+                    va[m] = cmul_i(va[m], sym_dx<R>(a.g, c, s) * a.scale);
+                } else if constexpr (OP == OP_DY) {
+                    va[m] = cmul_i(va[m], sym_dy<R>(a.g, s) * a.scale);
+                } else if constexpr (OP == OP_MASK) {
+                    va[m] = cscale(va[m], sym_mask<R>(a.g, c, s) * a.scale);
+                }
+            }
+            F::template run<+1>(va, buf1[0], t, tw);
+            store(a.outA, va);
         }
-        F::template run<+1>(vb, buf, t, tw);
-        if (live) {
-            C* dst = a.outB + (size_t)line * pitch;
-#pragma unroll
-            for (int m = 0; m < E; ++m) dst[t + T * m] = vb[m];
-        }
-    } else if (OP == OP_FILT) {
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int s = signed_k(t + T * m, a.g.ny);
-            R sg = sym_filter<R>(a.g, c, s, a.fstrength, a.forder, a.fcutoff, a.inv_kmax);
-            if (a.fdelta) sg -= (R)1;
-            va[m] = cscale(va[m], sg * a.scale);
-        }
-    } else if (OP == OP_DX || OP == OP_DY || OP == OP_MASK) {
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int s = signed_k(t + T * m, a.g.ny);
-            if (OP == OP_DX) va[m] = cmul_i(va[m], sym_dx<R>(a.g, c, s) * a.scale);
-            if (OP == OP_DY) va[m] = cmul_i(va[m], sym_dy<R>(a.g, s) * a.scale);
-            if (OP == OP_MASK) va[m] = cscale(va[m], sym_mask<R>(a.g, c, s) * a.scale);
-        }
-    }
-    F::template run<+1>(va, buf, t, tw);
-    if (live) {
-        C* dst = a.outA + (size_t)line * pitch;
-#pragma unroll
-        for (int m = 0; m < E; ++m) dst[t + T * m] = va[m];
     }
 }
 
@@ -641,20 +631,26 @@ __global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
     if constexpr (!STATE) return;
     if (!a.emit) return;  // uniform across the grid
 
-    // next-stage fluxes of both rows: x-FFT of (u*b) + i(v*b)
+    // next-stage fluxes of both rows, x-FFT of (u*b) + i(v*b), the two rows
+    // in lockstep: row A in buffer l, row B in buffer G + l, so buffer j holds
+    // row y0 + j and one store pass covers all 2G rows of the block
+    static_assert(MODE == MODE_PAIR || !STATE, "state epilogues pair rows");
+    C vv[2][E];
 #pragma unroll
-    for (int half = 0; half < 2; ++half) {
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const C uv = half == 0 ? pwA[m] : pwB[m];
-            const R s = half == 0 ? sA[m] : sB[m];
-            v[m] = C{uv.x * s, uv.y * s};
-        }
-        F::template run<-1>(v, buf, t, tw);
-        F::to_lds(v, buf, t);
-        const int r0 = half == 0 ? y0 : (MODE == MODE_PAIR ? y0 + G : y0);
-        store_split<R, N, E>(lds, G, a.ncols_out, a.Fx, a.Fy, pitch, r0, r0);
+    for (int m = 0; m < E; ++m) {
+        vv[0][m] = C{pwA[m].x * sA[m], pwA[m].y * sA[m]};
+        vv[1][m] = C{pwB[m].x * sB[m], pwB[m].y * sB[m]};
     }
+    C* const bb[2] = {lds + l * LE, lds + (G + l) * LE};
+    F::template run_n<-1, 2>(vv, bb, t, tw);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        bb[0][lpad(t + T * m)] = vv[0][m];
+        bb[1][lpad(t + T * m)] = vv[1][m];
+    }
+    __syncthreads();
+    store_split<R, N, E>(lds, 2 * G, a.ncols_out, a.Fx, a.Fy, pitch, y0, y0);
 }
 
 // ---------------------------------------------------------------------------

@@ -45,7 +45,7 @@ __device__ __forceinline__ void ma_load(const cpx<R>* P, int pitch, int y0, int 
 
 template <typename R, int N, int E, int MAXIT>
 __device__ __forceinline__ void ma_scatter(cpx<R>* lds, int G, int total, const cpx<R> (&q0)[MAXIT],
-                                           const cpx<R> (&q1)[MAXIT]) {
+                                           const cpx<R> (&q1)[MAXIT], int lstride) {
     using C = cpx<R>;
     constexpr int LE = lds_line_elems(N);
 #pragma unroll
@@ -54,7 +54,7 @@ __device__ __forceinline__ void ma_scatter(cpx<R>* lds, int G, int total, const 
         if (idx < total) {
             const int kx = idx / G, ll = idx - kx * G;
             C p0 = q0[it], p1 = q1[it];
-            C* bl = lds + ll * LE;
+            C* bl = lds + ll * lstride * LE;
             const bool edge = (kx == 0) || (2 * kx == N);
             if (edge) {
                 p0.y = 0;
@@ -83,42 +83,34 @@ __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
     const int total = a.ncols * G;
     typename F::Twiddles tw;
     tw.load(t, a.twx);
-    C* buf = lds + l * LE;
-    C q0[MAXIT], q1[MAXIT];
-    R xxA[E], xxB[E], yyA[E], yyB[E];
-    C v[E];
-    ma_load<R, N, E, MAXIT>(a.Pxx, pitch, y0, G, total, q0, q1);
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-        __syncthreads();  // previous transform's LDS reads are done
-        ma_scatter<R, N, E, MAXIT>(lds, G, total, q0, q1);
-        if (f == 0) ma_load<R, N, E, MAXIT>(a.Pyy, pitch, y0, G, total, q0, q1);
-        if (f == 1) ma_load<R, N, E, MAXIT>(a.Pxy, pitch, y0, G, total, q0, q1);
-        F::template run_from_lds<+1>(v, buf, t, tw);
-        if (f == 0) {
-#pragma unroll
-            for (int m = 0; m < E; ++m) {
-                xxA[m] = v[m].x;
-                xxB[m] = v[m].y;
-            }
-        } else if (f == 1) {
-#pragma unroll
-            for (int m = 0; m < E; ++m) {
-                yyA[m] = v[m].x;
-                yyB[m] = v[m].y;
-            }
-        }
-    }
+    // the three fields go to buffers 3l, 3l+1, 3l+2 and are inverted in
+    // lockstep; all global loads are issued before the first LDS store
+    C* const b3[3] = {lds + (3 * l) * LE, lds + (3 * l + 1) * LE, lds + (3 * l + 2) * LE};
+    C qa0[MAXIT], qa1[MAXIT], qb0[MAXIT], qb1[MAXIT], qc0[MAXIT], qc1[MAXIT];
+    ma_load<R, N, E, MAXIT>(a.Pxx, pitch, y0, G, total, qa0, qa1);
+    ma_load<R, N, E, MAXIT>(a.Pyy, pitch, y0, G, total, qb0, qb1);
+    ma_load<R, N, E, MAXIT>(a.Pxy, pitch, y0, G, total, qc0, qc1);
+    ma_scatter<R, N, E, MAXIT>(lds, G, total, qa0, qa1, 3);
+    ma_scatter<R, N, E, MAXIT>(lds + LE, G, total, qb0, qb1, 3);
+    ma_scatter<R, N, E, MAXIT>(lds + 2 * LE, G, total, qc0, qc1, 3);
+    C w3[3][E];
+    F::template run_n_from_lds<+1, 3>(w3, b3, t, tw);
     // D = Φxx Φyy - Φxy² for rows A (re) and B (im): compute_d_operator!
     // (src/poisson.jl:183-223)
+    C v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-        const R xyA = v[m].x, xyB = v[m].y;
-        v[m] = C{xxA[m] * yyA[m] - xyA * xyA, xxB[m] * yyB[m] - xyB * xyB};
+        const C xx = w3[0][m], yy = w3[1][m], xy = w3[2][m];
+        v[m] = C{xx.x * yy.x - xy.x * xy.x, xx.y * yy.y - xy.y * xy.y};
     }
+    C* buf = b3[0];
     F::template run<-1>(v, buf, t, tw);
-    F::to_lds(v, buf, t);
-    store_split<R, N, E>(lds, G, a.ncols, a.Dx, a.Dx, pitch, y0, y0 + G);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) buf[lpad(t + T * m)] = v[m];
+    __syncthreads();
+    // line l's result sits in buffer 3l: store_split with a stride-3 view
+    store_split_stride<R, N, E>(lds, G, 3, a.ncols, a.Dx, a.Dx, pitch, y0, y0 + G);
 }
 
 // Per-level stencil data, 0-based levels, nz >= 3 (src/transforms.jl:798-850,
