@@ -105,6 +105,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB three times with a plain device copy kernel "
+                         "(reference point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -177,6 +180,12 @@ def main():
         log("bench: CPU baseline (oracle)")
         dt = args.dt
         out["cpu_baseline"] = cpu_baseline(b0, psi, nx, ny, nz, dt, args.cpu_seconds)
+    if args.copy_calibration:
+        d = ssg.make_domain(64, 64, 1)
+        nbytes = args.copy_calibration * (1 << 20)
+        d.check(d.lib.ssg_copy_probe(d.handle, nbytes, 3))
+        d.close()
+        out["copy_calibration_bytes"] = nbytes
     print(json.dumps(out), flush=True)
 
 

@@ -194,6 +194,12 @@ int ssg_profile(ssg_ctx* ctx, int32_t enable);
    candidates 4, 8, 16 per class (-1 where not timed).  Tuning runs on the
    first ssg_timestep of a path. */
 int ssg_tuning(ssg_ctx* ctx, int32_t path, int32_t* ept, double* us);
+
+/* Measurement aid with no reference counterpart: runs a plain device copy
+   kernel (k_convert<double,double>) of `bytes` bytes `reps` times, the
+   calibration point for hardware byte counters (rocprofv3 FETCH_SIZE /
+   WRITE_SIZE). */
+int ssg_copy_probe(ssg_ctx* ctx, int64_t bytes, int32_t reps);
 int ssg_profile_read(ssg_ctx* ctx, int32_t kind, double* total_ms, int64_t* count, double* bytes);
 
 #ifdef __cplusplus

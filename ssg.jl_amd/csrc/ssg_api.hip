@@ -1314,6 +1314,30 @@ int ssg_set_coupling(ssg_ctx* c, const ssg_ma_params* p) {
     return SSG_OK;
 }
 
+int ssg_copy_probe(ssg_ctx* c, int64_t bytes, int32_t reps) {
+    if (!c || bytes <= 0 || reps < 1) return SSG_ERR_ARG;
+    HIP_OK(c, hipSetDevice(c->device));
+    const long n = (long)(bytes / (int64_t)sizeof(double));
+    double *a = nullptr, *b = nullptr;
+    HIP_OK(c, hipMalloc(&a, n * sizeof(double)));
+    if (hipMalloc(&b, n * sizeof(double)) != hipSuccess) {
+        (void)hipFree(a);
+        return set_err(c, SSG_ERR_HIP, "copy probe allocation failed");
+    }
+    long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipError_t e = hipMemsetAsync(a, 0, n * sizeof(double), c->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(blocks), dim3(256), 0, c->stream, a, b, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return set_err(c, SSG_ERR_HIP, "copy probe: %s", hipGetErrorString(e));
+    return SSG_OK;
+}
+
 int ssg_last_ma_diag(ssg_ctx* c, ssg_ma_diag* d) {
     if (!c || !d) return SSG_ERR_ARG;
     *d = c->last_diag;

@@ -50,7 +50,7 @@ EXPORTED_SYMBOLS = (
     "ssg_set_field_device", "ssg_get_field_device", "ssg_timestep", "ssg_tendency", "ssg_jacobian",
     "ssg_rfft_2d", "ssg_irfft_2d", "ssg_apply_exponential_filter", "ssg_diagnostics", "ssg_last_timing",
     "ssg_profile", "ssg_profile_read", "ssg_tuning", "ssg_ma_solve", "ssg_get_phi3d", "ssg_set_coupling",
-    "ssg_last_ma_diag",
+    "ssg_last_ma_diag", "ssg_copy_probe",
 )
 
 MA_BODY = 0      # as wired: b on every level, surface slope 0
@@ -149,6 +149,7 @@ def load_library(path: str | None = None):
     lib.ssg_get_phi3d.argtypes = [vp, _dp, ctypes.c_int64]
     lib.ssg_set_coupling.argtypes = [vp, ctypes.POINTER(_MaParams)]
     lib.ssg_last_ma_diag.argtypes = [vp, ctypes.POINTER(_MaDiag)]
+    lib.ssg_copy_probe.argtypes = [vp, ctypes.c_int64, ctypes.c_int32]
     lib.ssg_profile_read.argtypes = [vp, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), _dp]
     for n in EXPORTED_SYMBOLS:
         getattr(lib, n).restype = ctypes.c_char_p if n == "ssg_last_error" else ctypes.c_int
