@@ -82,6 +82,9 @@ struct ssg_ctx {
     ssg_ma_params ma_par{};
     double* bst = nullptr;            // stage buoyancy (coupled mode)
     ssg_ma_diag last_diag{};
+    double* adj_lam = nullptr;        // adjoint state
+    double* adj_acc = nullptr;
+    double* adj_s = nullptr;
     struct GraphCache {
         hipGraphExec_t exec = nullptr;
         int len = 0;
@@ -424,9 +427,10 @@ int run_kind(ssg_ctx* c, int kind) {
 // apply_spectral_filter! on b; FP64 sets b = irfft(σ b̂); FP32-mixed adds
 // the FP32 defect irfft((σ-1) b̂) to the FP64 b.  emit: refresh fluxes.
 template <typename R>
-int filter_b(ssg_ctx* c, double strength, int order, double cutoff, int emit) {
+int filter_b(ssg_ctx* c, double strength, int order, double cutoff, int emit, double* target = nullptr) {
     PathBuffers<R>& B = bufs<R>(c);
-    int rc = launch_rows_fwd<R, double, SRC_PAIR>(c, c->b, nullptr, B.Tp, nullptr, c->ncols_full);
+    double* tb = target ? target : c->b;
+    int rc = launch_rows_fwd<R, double, SRC_PAIR>(c, tb, nullptr, B.Tp, nullptr, c->ncols_full);
     if (rc) return rc;
     const int delta = std::is_same<R, float>::value ? 1 : 0;
     rc = launch_cols<R, OP_FILT>(c, B.Tp, nullptr, B.Tp, nullptr, c->ncols_full, (R)(1.0 / ((double)c->nx * c->ny)),
@@ -435,7 +439,7 @@ int filter_b(ssg_ctx* c, double strength, int order, double cutoff, int emit) {
     RowsInvArgs<R> a{};
     a.P0 = B.Tp;
     a.ncols_in = c->ncols_full;
-    a.b = c->b;
+    a.b = tb;
     a.emit = emit;
     a.w = B.w;
     a.Fx = B.Fx;
@@ -923,6 +927,66 @@ int ma_solve_dev(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_di
     return SSG_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of timestep! with frozen φₛ (FP64).  The step is linear in
+// b, so its transpose needs no trajectory: for RK3
+//   s3 = λ:              g3 = (3 dt/4) Tᵀ s3,  acc = λ + g3,  s2 = (2dt/3) g3
+//   s2:                  g2 = Tᵀ s2,           acc += g2,     s1 = (dt/4) λ + (dt/3) g2
+//   s1:                  g1 = Tᵀ s1,           λ ← acc + g1
+// with Tᵀ s = u Dx(M s) + v Dy(M s).  The filter E[σ] (real, even σ) is
+// self-adjoint and is applied to λ before the RK3 transpose of a filter step.
+// ---------------------------------------------------------------------------
+int adj_alloc(ssg_ctx* c) {
+    if (c->adj_lam) return SSG_OK;
+    const size_t n = (size_t)c->nx * c->ny;
+    HIP_OK(c, hipMalloc(&c->adj_lam, n * sizeof(double)));
+    HIP_OK(c, hipMalloc(&c->adj_acc, n * sizeof(double)));
+    HIP_OK(c, hipMalloc(&c->adj_s, n * sizeof(double)));
+    return SSG_OK;
+}
+
+template <int EPI>
+int adj_stage(ssg_ctx* c, const double* sin, double gscale, double dt) {
+    PathBuffers<double>& D = c->d;
+    int rc = launch_rows_fwd<double, double, SRC_PAIR>(c, sin, nullptr, D.Tp, nullptr, c->ncols_tend);
+    if (rc) return rc;
+    ColsArgs<double> a{};
+    a.inA = D.Tp;
+    a.outA = D.Fx;
+    a.outB = D.Fy;
+    a.scale = 1.0 / ((double)c->nx * c->ny);
+    if ((rc = launch_cols_args<double, OP_ADJ>(c, a, c->ncols_tend, 1))) return rc;
+    RowsInvArgs<double> r{};
+    r.P0 = D.Fx;
+    r.P1 = D.Fy;
+    r.ncols_in = c->ncols_tend;
+    r.w = D.w;
+    r.lam = c->adj_lam;
+    r.acc = c->adj_acc;
+    r.sbuf = c->adj_s;
+    r.c1 = dt / 3.0;
+    r.c2 = 2.0 * dt / 3.0;
+    r.c3 = dt / 4.0;
+    r.gscale = gscale;
+    return launch_rows_inv<double, MODE_PLANES, EPI>(c, r);
+}
+
+int adjoint_steps(ssg_ctx* c, const ssg_time_params* p, int64_t step_end, int64_t nsteps) {
+    const double dt = p->dt;
+    int rc;
+    for (int64_t n = step_end; n > step_end - nsteps; --n) {
+        // step n took b from step n-1 to n; the filter ran after it when n % F == 0
+        if (p->filter_freq > 0 && (n % p->filter_freq) == 0)
+            if ((rc = filter_b<double>(c, p->filter_strength, p->filter_order, p->filter_cutoff, 0, c->adj_lam)))
+                return rc;
+        if ((rc = adj_stage<EPI_ADJ3>(c, c->adj_lam, 3.0 * dt / 4.0, dt))) return rc;
+        if ((rc = adj_stage<EPI_ADJ2>(c, c->adj_s, 1.0, dt))) return rc;
+        if ((rc = adj_stage<EPI_ADJ1>(c, c->adj_s, 1.0, dt))) return rc;
+    }
+    return SSG_OK;
+}
+
 int check_domain(const ssg_domain_desc* d, char* msg, size_t len) {
     if (!d) return snprintf(msg, len, "null domain descriptor"), SSG_ERR_ARG;
     if (!pow2_in_range(d->nx) || !pow2_in_range(d->ny))
@@ -1102,6 +1166,9 @@ int ssg_destroy(ssg_ctx* c) {
     free_path(c->d);
     free_path(c->f);
     ma_free(c);
+    (void)hipFree(c->adj_lam);
+    (void)hipFree(c->adj_acc);
+    (void)hipFree(c->adj_s);
     (void)hipFree(c->b);
     (void)hipFree(c->phi);
     (void)hipFree(c->dscratch);
@@ -1335,6 +1402,25 @@ int ssg_copy_probe(ssg_ctx* c, int64_t bytes, int32_t reps) {
     (void)hipFree(a);
     (void)hipFree(b);
     if (e != hipSuccess) return set_err(c, SSG_ERR_HIP, "copy probe: %s", hipGetErrorString(e));
+    return SSG_OK;
+}
+
+int ssg_adjoint(ssg_ctx* c, const ssg_time_params* p, int64_t step_end, int64_t nsteps, const double* lam_in,
+                double* lam_out) {
+    if (!c) return SSG_ERR_ARG;
+    if (!p || !lam_in || !lam_out) return set_err(c, SSG_ERR_ARG, "null pointer");
+    if (p->scheme != SSG_RK3) return set_err(c, SSG_ERR_UNSUPPORTED, "the adjoint covers the RK3 scheme");
+    if (c->coupled) return set_err(c, SSG_ERR_UNSUPPORTED, "the adjoint covers the frozen-φₛ step");
+    if (nsteps < 0 || step_end < nsteps) return set_err(c, SSG_ERR_ARG, "need 0 <= nsteps <= step_end");
+    if (!(p->dt > 0)) return set_err(c, SSG_ERR_ARG, "dt must be > 0");
+    HIP_OK(c, hipSetDevice(c->device));
+    int rc = adj_alloc(c);
+    if (rc) return rc;
+    const size_t bytes = (size_t)c->nx * c->ny * sizeof(double);
+    HIP_OK(c, hipMemcpyAsync(c->adj_lam, lam_in, bytes, hipMemcpyHostToDevice, c->stream));
+    if ((rc = adjoint_steps(c, p, step_end, nsteps))) return rc;
+    HIP_OK(c, hipMemcpyAsync(lam_out, c->adj_lam, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
     return SSG_OK;
 }
 

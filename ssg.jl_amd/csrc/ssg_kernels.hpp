@@ -32,7 +32,12 @@ enum {
     EPI_RK3 = 4,
     EPI_AB2 = 5,
     EPI_SET = 6,     // b = value        (filter, FP64)
-    EPI_ADD = 7      // b += value       (filter, defect form)
+    EPI_ADD = 7,     // b += value       (filter, defect form)
+    // discrete adjoint of the RK3 stage chain (MODE_PLANES: the two planes are
+    // Dx M s and Dy M s of the stage input s; g = gscale (u Gx + v Gy) = gscale T^T s)
+    EPI_ADJ3 = 8,    // acc = lam + g;  s = c2 g
+    EPI_ADJ2 = 9,    // acc += g;       s = c3 lam + c1 g
+    EPI_ADJ1 = 10    // lam = acc + g
 };
 enum {
     OP_TEND = 0,     // -M (Dx Fx + Dy Fy)
@@ -47,7 +52,8 @@ enum {
     OP_MASK = 9,     // M P
     OP_MA_DERIV = 10,  // Monge-Ampere: P -> (Pxx, Pyy, Pxy), P spectral
     OP_MA_DIFF = 11,   // Monge-Ampere: y-FFT of D, residual sums against the previous D
-    OP_VEL_SPEC = 12   // U = -Dy P, V = Dx P with P already spectral in y
+    OP_VEL_SPEC = 12,  // U = -Dy P, V = Dx P with P already spectral in y
+    OP_ADJ = 13        // A = Dx M P, B = Dy M P (transposed tendency, spectral part)
 };
 
 struct Grid {
@@ -282,7 +288,8 @@ __device__ __forceinline__ void block_sum_g(double (&v)[W], double* out) {
 // LDS line buffers per column line for each operation
 __host__ __device__ constexpr int cols_nbuf(int op) {
     return op == OP_MA_DERIV ? 3
-                             : ((op == OP_TEND || op == OP_TEND_Q || op == OP_JAC || op == OP_VEL || op == OP_VEL_SPEC)
+                             : ((op == OP_TEND || op == OP_TEND_Q || op == OP_JAC || op == OP_VEL || op == OP_VEL_SPEC ||
+                                 op == OP_ADJ)
                                     ? 2
                                     : 1);
 }
@@ -390,6 +397,21 @@ __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
             }
             block_sum_g<2>(acc, a.partial);
             return;
+        } else if constexpr (OP == OP_ADJ) {
+            // transpose of the tendency's spectral part: the symbols are
+            // Hermitian, so E[S]^T = E[conj S]; M is real, Dx and Dy are
+            // imaginary: (-M (Dx U + Dy V))^T = (U Dx + V Dy) M
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const int s = signed_k(t + T * m, a.g.ny);
+                const C p = cscale(va[m], sym_mask<R>(a.g, c, s) * a.scale);
+                vb[m] = cmul_i(p, sym_dy<R>(a.g, s));
+                va[m] = cmul_i(p, sym_dx<R>(a.g, c, s));
+            }
+            F::template run_n<+1, 2>(vv, b2, t, tw);
+            store(a.outA, va);
+            store(a.outB, vb);
+            return;
         } else if constexpr (OP == OP_VEL || OP == OP_VEL_SPEC) {
 #pragma unroll
             for (int m = 0; m < E; ++m) {
@@ -458,7 +480,11 @@ struct RowsInvArgs {
     R* out_r;              // EPI_STORE
     cpx<R>* w_out;         // EPI_STORE_W
     double* bstage;        // RK1/RK2: also store the stage buoyancy (coupled mode)
-    double c1, c2;         // stage coefficients
+    double* lam;           // adjoint: the adjoint state
+    double* acc;           // adjoint: accumulated lambda of the step
+    double* sbuf;          // adjoint: next stage's input
+    double c1, c2, c3;     // stage coefficients
+    double gscale;         // adjoint: scale of T^T s
     int first;             // AB2: forward Euler on step 0
     // next-stage flux emission
     int emit;
@@ -491,7 +517,7 @@ __global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
     tw.load(t, a.twx);
 
     // epilogue operands do not depend on the transform: load them first
-    constexpr bool STATE = (EPI >= EPI_RK1);
+    constexpr bool STATE = (EPI >= EPI_RK1 && EPI <= EPI_ADD);
     constexpr bool NEED_B = (EPI == EPI_RK1 || EPI == EPI_RK2 || EPI == EPI_RK3 || EPI == EPI_AB2 || EPI == EPI_ADD);
     double pbA[STATE ? E : 1], pbB[STATE ? E : 1];
     R pkA[STATE ? E : 1], pkB[STATE ? E : 1];
@@ -577,7 +603,20 @@ __global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
     for (int m = 0; m < E; ++m) {
         const int x = t + T * m;
         const R tA = v[m].x, tB = v[m].y;
-        if (EPI == EPI_STORE) {
+        if constexpr (EPI == EPI_ADJ3 || EPI == EPI_ADJ2 || EPI == EPI_ADJ1) {
+            // MODE_PLANES: tA = (Dx M s)(x), tB = (Dy M s)(x) of row A
+            const C uv = a.w[bA + x];
+            const double g = a.gscale * ((double)uv.x * (double)tA + (double)uv.y * (double)tB);
+            if (EPI == EPI_ADJ3) {
+                a.acc[bA + x] = a.lam[bA + x] + g;
+                a.sbuf[bA + x] = a.c2 * g;
+            } else if (EPI == EPI_ADJ2) {
+                a.acc[bA + x] += g;
+                a.sbuf[bA + x] = a.c3 * a.lam[bA + x] + a.c1 * g;
+            } else {
+                a.lam[bA + x] = a.acc[bA + x] + g;
+            }
+        } else if (EPI == EPI_STORE) {
             a.out_r[bA + x] = tA;
             a.out_r[bB + x] = tB;
         } else if (EPI == EPI_STORE_W) {

@@ -50,7 +50,7 @@ EXPORTED_SYMBOLS = (
     "ssg_set_field_device", "ssg_get_field_device", "ssg_timestep", "ssg_tendency", "ssg_jacobian",
     "ssg_rfft_2d", "ssg_irfft_2d", "ssg_apply_exponential_filter", "ssg_diagnostics", "ssg_last_timing",
     "ssg_profile", "ssg_profile_read", "ssg_tuning", "ssg_ma_solve", "ssg_get_phi3d", "ssg_set_coupling",
-    "ssg_last_ma_diag", "ssg_copy_probe",
+    "ssg_last_ma_diag", "ssg_copy_probe", "ssg_adjoint",
 )
 
 MA_BODY = 0      # as wired: b on every level, surface slope 0
@@ -150,6 +150,7 @@ def load_library(path: str | None = None):
     lib.ssg_set_coupling.argtypes = [vp, ctypes.POINTER(_MaParams)]
     lib.ssg_last_ma_diag.argtypes = [vp, ctypes.POINTER(_MaDiag)]
     lib.ssg_copy_probe.argtypes = [vp, ctypes.c_int64, ctypes.c_int32]
+    lib.ssg_adjoint.argtypes = [vp, ctypes.POINTER(_TimeParams), ctypes.c_int64, ctypes.c_int64, _dp, _dp]
     lib.ssg_profile_read.argtypes = [vp, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), _dp]
     for n in EXPORTED_SYMBOLS:
         getattr(lib, n).restype = ctypes.c_char_p if n == "ssg_last_error" else ctypes.c_int
@@ -436,3 +437,14 @@ def set_coupling(domain: Domain, *, eps: float | None = 0.1, tol: float = 1e-10,
         return
     p = _ma_params(eps, tol, maxiter, variant, warm)
     domain.check(domain.lib.ssg_set_coupling(domain.handle, ctypes.byref(p)))
+
+
+def timestep_adjoint(domain: Domain, params: TimeParams, step_end: int, nsteps: int, lam) -> np.ndarray:
+    """Vector-Jacobian product of ``nsteps`` RK3 ``timestep!`` calls (frozen φₛ)
+    that ended at step ``step_end``: returns (∂b_end/∂b_start)ᵀ lam."""
+    lam = _f64(lam, (domain.Ny, domain.Nx))
+    out = np.empty_like(lam)
+    p = params._c()
+    domain.check(domain.lib.ssg_adjoint(domain.handle, ctypes.byref(p), int(step_end), int(nsteps),
+                                        lam.ctypes.data_as(_dp), out.ctypes.data_as(_dp)))
+    return out
