@@ -93,6 +93,73 @@ def cpu_baseline(b0, psi, nx, ny, nz, dt, budget_s):
             "sample": f"{n} RK3 steps of the same {nx}x{ny} workload (FP64 C oracle, OpenMP), {el:.1f} s"}
 
 
+def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec):
+    """Coupled step (Monge-Ampère solve at every RK stage, body-forced
+    variant, ε = 0.1, relative residual 1e-10): steps/s plus the Monge-Ampère
+    solve's iterations, time and per-kernel-class profile."""
+    import ctypes
+    d = ssg.make_domain(nx, ny, nz, precision=prec)
+    f = ssg.allocate_fields(d)
+    f.bs = b0
+    diag = ssg.solve_monge_ampere(f, d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY)
+    cold = {"iterations": diag["iterations"], "ms": diag["solve_time"] * 1e3,
+            "relative_residual": diag["relative_residual"], "converged": diag["converged"]}
+    d.profile(True)
+    ssg.solve_monge_ampere(f, d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY)
+    names = {1: "k_cols (MA ops)", 2: "k_rows_inv", 3: "k_ma_rows", 4: "k_ma_tridiag", 0: "k_rows_fwd"}
+    kernels = {}
+    for k, nm in names.items():
+        ms, n, by = d.profile_read(k)
+        if n:
+            kernels[nm] = {"avg_us": 1e3 * ms / n, "launches": n, "GBps": by / max(ms, 1e-12) / 1e6}
+    d.profile(False)
+    ssg.set_coupling(d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY, warm=True)
+    st = ssg.TimeState(0.0, f)
+    params = ssg.TimeParams(dt, scheme=ssg.RK3, filter_freq=10, filter_strength=36.0, filter_order=8,
+                            filter_cutoff=0.65)
+    ssg.timestep(f, d, params, st, 1)
+    d.sync()
+    t0 = time.perf_counter()
+    ssg.timestep(f, d, params, st, steps)
+    d.sync()
+    wall = time.perf_counter() - t0
+    last = ssg._MaDiag()
+    d.lib.ssg_last_ma_diag(d.handle, ctypes.byref(last))
+    d.close()
+    return {"steps_per_s": steps / wall, "steps": steps, "variant": "body (as wired)", "eps": 0.1, "tol": 1e-10,
+            "cold_solve": cold, "warm_solve_last_iterations": last.iterations,
+            "warm_solve_last_relative_residual": last.relative_residual, "ma_kernels": kernels}
+
+
+def fft_compare(ssg, n, reps):
+    """Spectral round trip at n x n: this library's filter application
+    (k_rows_fwd + k_cols + k_rows_inv, HIP-event kernel times summed) against
+    rocFFT R2C + scale + C2R (HIP events around back-to-back round trips)."""
+    import ctypes
+    out = {"n": n, "reps": reps}
+    lib = os.path.join(ROOT, "ssg.jl_amd", "lib", "libssg_rocfft_cmp.so")
+    cmp = ctypes.CDLL(lib)
+    cmp.ssg_cmp_rocfft.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_double)]
+    for name, prec in (("fp64", ssg.SSG_FP64), ("fp32", ssg.SSG_FP32_MIXED)):
+        d = ssg.make_domain(n, n, 1, precision=prec)
+        f = ssg.allocate_fields(d)
+        f.bs = np.random.default_rng(0).standard_normal((n, n))
+        for _ in range(5):
+            ssg.apply_spectral_filter(f, d, 36.0, 8, 0.65)
+        d.profile(True)
+        for _ in range(reps):
+            ssg.apply_spectral_filter(f, d, 36.0, 8, 0.65)
+        ours = sum(d.profile_read(k)[0] for k in KIND_NAMES) * 1e3 / reps
+        d.profile(False)
+        d.close()
+        us = (ctypes.c_double * 3)()
+        rc = cmp.ssg_cmp_rocfft(n, n, 1 if name == "fp32" else 0, reps, us)
+        out[name] = {"ours_roundtrip_us": ours, "rocfft_roundtrip_us": us[0] if rc == 0 else None,
+                     "rocfft_fwd_us": us[1] if rc == 0 else None, "rocfft_inv_us": us[2] if rc == 0 else None}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +172,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None)
+    ap.add_argument("--coupled-steps", type=int, default=3,
+                    help="also time this many coupled steps (Monge-Ampère solve per stage); 0 = skip")
+    ap.add_argument("--fft-compare", action="store_true",
+                    help="only time the spectral round trip (rfft2 -> diagonal -> irfft2) of this library "
+                         "against rocFFT through hipFFT, at --nx")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB three times with a plain device copy kernel "
                          "(reference point for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -118,6 +190,9 @@ def main():
     from ssgamd.ics import spectral_b0, streamfunction_from_b
 
     nx = ny = args.nx
+    if args.fft_compare:
+        print(json.dumps(fft_compare(ssg, nx, reps=200)), flush=True)
+        return
     nz = args.nz
     b0 = spectral_b0(nx, ny, seed=0, k0=6.0)
     psi = streamfunction_from_b(b0)
@@ -176,6 +251,9 @@ def main():
                               "GBps": v["bytes"] / max(v["ms"], 1e-12) / 1e6} for k, v in kinds.items()}
         out["step_algorithmic_GBps"] = tot_b / (tot_ms / 1e3) / 1e9 if tot_ms > 0 else None
     out["autotune"] = {k: v[5] for k, v in res.items()}
+    if args.coupled_steps > 0:
+        log(f"bench: coupled RK3 {nx}x{ny}x{nz}, {args.coupled_steps} steps")
+        out["coupled_fp64"] = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP64)
     if not args.no_cpu:
         log("bench: CPU baseline (oracle)")
         dt = args.dt
