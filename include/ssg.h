@@ -210,6 +210,28 @@ int ssg_tuning(ssg_ctx* ctx, int32_t path, int32_t* ept, double* us);
 int ssg_copy_probe(ssg_ctx* ctx, int64_t bytes, int32_t reps);
 int ssg_profile_read(ssg_ctx* ctx, int32_t kind, double* total_ms, int64_t* count, double* bytes);
 
+/* ------------------------------------------------------------------------
+   Ensemble (batched form): members with their own grids, advanced together
+   with one launch per kernel class per RK stage.  No reference counterpart:
+   the reference runs one domain per MPI job.
+   ------------------------------------------------------------------------ */
+typedef struct ssg_ensemble ssg_ensemble;
+
+/* members share precision and device; nx, ny in {128, 256, 512, 1024} */
+int ssg_ensemble_create(ssg_ensemble** out, int32_t nmembers, const ssg_domain_desc* members);
+int ssg_ensemble_destroy(ssg_ensemble* ens);
+/* member context (owned by the ensemble): set fields, solve Monge-Ampère,
+   read diagnostics through the single-domain calls */
+ssg_ctx* ssg_ensemble_member(ssg_ensemble* ens, int32_t index);
+const char* ssg_ensemble_last_error(const ssg_ensemble* ens);
+/* nsteps RK3 timestep! calls (frozen φₛ) of every member; dt[i] per member
+   (params->dt is ignored), states[i] updated; members at the same step */
+int ssg_ensemble_timestep(ssg_ensemble* ens, const ssg_time_params* params, const double* dt, ssg_time_state* states,
+                          int64_t nsteps);
+/* autotuned elements per thread of the batched launches and the candidate
+   timings (us per RK stage pair, -1 where not valid) for 4, 8, 16 */
+int ssg_ensemble_tuning(ssg_ensemble* ens, int32_t* ept, double* us);
+
 #ifdef __cplusplus
 }
 #endif

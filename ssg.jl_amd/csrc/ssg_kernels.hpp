@@ -195,7 +195,7 @@ __device__ __forceinline__ void store_split_stride(cpx<R>* lds, int G, int lstri
 }
 
 template <typename R, typename S, int N, int E, int SRC>
-__global__ __launch_bounds__(256) void k_rows_fwd(RowsFwdArgs<R, S> a) {
+__device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs<R, S>& a, const int blk) {
     using C = cpx<R>;
     using F = LineFFT<R, N, E>;
     constexpr int T = F::T;
@@ -204,7 +204,6 @@ __global__ __launch_bounds__(256) void k_rows_fwd(RowsFwdArgs<R, S> a) {
     C* lds = reinterpret_cast<C*>(smem_raw);
     const int l = threadIdx.x / T, t = threadIdx.x - l * T;
     const int G = a.G;
-    const int blk = xcd_block(blockIdx.x, a.nblk);
     const int nx = a.g.nx;
     typename F::Twiddles tw;
     tw.load(t, a.twx);
@@ -237,6 +236,11 @@ __global__ __launch_bounds__(256) void k_rows_fwd(RowsFwdArgs<R, S> a) {
     store_split<R, N, E>(lds, G, a.ncols, a.PA, SRC == SRC_FLUX ? a.PB : a.PA, a.g.pitch, rowA0, rowB0);
 }
 
+template <typename R, typename S, int N, int E, int SRC>
+__global__ __launch_bounds__(256) void k_rows_fwd(RowsFwdArgs<R, S> a) {
+    rows_fwd_body<R, S, N, E, SRC>(a, xcd_block(blockIdx.x, a.nblk));
+}
+
 // ---------------------------------------------------------------------------
 // Column pass: per kx column, y-FFT -> symbol -> inverse y-FFT.
 // ---------------------------------------------------------------------------
@@ -266,7 +270,7 @@ struct ColsArgs {
 // per-block sums of W values into out[blockIdx.x * W + k]; blockDim is a
 // power of two <= 256 (tree reduction in LDS, no partial-wave shuffles)
 template <int W>
-__device__ __forceinline__ void block_sum_g(double (&v)[W], double* out) {
+__device__ __forceinline__ void block_sum_g(double (&v)[W], double* out, int bidx) {
     __shared__ double red_g[W][256];
     const int tid = threadIdx.x, nt = blockDim.x;
 #pragma unroll
@@ -281,7 +285,7 @@ __device__ __forceinline__ void block_sum_g(double (&v)[W], double* out) {
     }
     if (tid == 0) {
 #pragma unroll
-        for (int k = 0; k < W; ++k) out[(size_t)blockIdx.x * W + k] = red_g[k][0];
+        for (int k = 0; k < W; ++k) out[(size_t)bidx * W + k] = red_g[k][0];
     }
 }
 
@@ -295,7 +299,7 @@ __host__ __device__ constexpr int cols_nbuf(int op) {
 }
 
 template <typename R, int M, int E, int OP>
-__global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
+__device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) {
     using C = cpx<R>;
     using F = LineFFT<R, M, E>;
     constexpr int T = F::T;
@@ -304,7 +308,7 @@ __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
     const int l = threadIdx.x / T, t = threadIdx.x - l * T;
-    const int line = blockIdx.x * a.G + l;
+    const int line = bidx * a.G + l;
     const bool live = line < a.ncols;
     const int pitch = a.g.ny;                         // lines are contiguous runs of ny
     const int c = a.nlev > 1 ? line / a.nlev : line;  // kx column of this line
@@ -395,7 +399,7 @@ __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
                     dp[y] = va[m];
                 }
             }
-            block_sum_g<2>(acc, a.partial);
+            block_sum_g<2>(acc, a.partial, bidx);
             return;
         } else if constexpr (OP == OP_ADJ) {
             // transpose of the tendency's spectral part: the symbols are
@@ -459,6 +463,11 @@ __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
     }
 }
 
+template <typename R, int M, int E, int OP>
+__global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
+    cols_body<R, M, E, OP>(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Row pass, inverse + epilogue.
 //   MODE_PAIR  : line l <-> rows (y0+l, y0+G+l) of plane P0
@@ -497,7 +506,7 @@ struct RowsInvArgs {
 };
 
 template <typename R, int N, int E, int MODE, int EPI>
-__global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
+__device__ __forceinline__ void rows_inv_body(const RowsInvArgs<R>& a, const int blk) {
     using C = cpx<R>;
     using F = LineFFT<R, N, E>;
     constexpr int T = F::T;
@@ -506,7 +515,6 @@ __global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
     C* lds = reinterpret_cast<C*>(smem_raw);
     const int l = threadIdx.x / T, t = threadIdx.x - l * T;
     const int G = a.G;
-    const int blk = xcd_block(blockIdx.x, a.nblk);
     const int pitch = a.g.pitch, nx = a.g.nx;
     const int rows_per_blk = (MODE == MODE_PAIR) ? 2 * G : G;
     const int y0 = blk * rows_per_blk;
@@ -692,6 +700,11 @@ __global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
     store_split<R, N, E>(lds, 2 * G, a.ncols_out, a.Fx, a.Fy, pitch, y0, y0);
 }
 
+template <typename R, int N, int E, int MODE, int EPI>
+__global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
+    rows_inv_body<R, N, E, MODE, EPI>(a, xcd_block(blockIdx.x, a.nblk));
+}
+
 // ---------------------------------------------------------------------------
 // Layout conversion between this library's spectrum (kx <= nx/2 planes,
 // full ky) and the reference's rfft2 array (nx, ny/2+1), x fastest.
@@ -813,6 +826,43 @@ __global__ void k_w_split(const cpx<R>* __restrict__ w, double* __restrict__ u, 
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
         u[q] = (double)w[q].x;
         v[q] = (double)w[q].y;
+    }
+}
+
+}  // namespace ssg
+
+namespace ssg {
+
+// ---------------------------------------------------------------------------
+// Batched (ensemble) launches: one launch covers every member; block b works
+// on work[b] = (member, block within that member) with the member's own
+// argument struct, and dispatches on the member's line length.  Every member
+// uses G = 256 / T lines per block, so the block shape is the same for all.
+// ---------------------------------------------------------------------------
+template <typename R, int E, int OP>
+__global__ __launch_bounds__(256) void k_cols_batch(const ColsArgs<R>* __restrict__ args, const int2* __restrict__ work) {
+    const int2 wk = work[blockIdx.x];
+    const ColsArgs<R>& a = args[wk.x];
+    switch (a.g.ny) {
+        case 128: cols_body<R, 128, E, OP>(a, wk.y); break;
+        case 256: cols_body<R, 256, E, OP>(a, wk.y); break;
+        case 512: cols_body<R, 512, E, OP>(a, wk.y); break;
+        case 1024: cols_body<R, 1024, E, OP>(a, wk.y); break;
+        default: break;
+    }
+}
+
+template <typename R, int E, int EPI>
+__global__ __launch_bounds__(256) void k_rows_inv_batch(const RowsInvArgs<R>* __restrict__ args,
+                                                        const int2* __restrict__ work) {
+    const int2 wk = work[blockIdx.x];
+    const RowsInvArgs<R>& a = args[wk.x];
+    switch (a.g.nx) {
+        case 128: rows_inv_body<R, 128, E, MODE_PAIR, EPI>(a, wk.y); break;
+        case 256: rows_inv_body<R, 256, E, MODE_PAIR, EPI>(a, wk.y); break;
+        case 512: rows_inv_body<R, 512, E, MODE_PAIR, EPI>(a, wk.y); break;
+        case 1024: rows_inv_body<R, 1024, E, MODE_PAIR, EPI>(a, wk.y); break;
+        default: break;
     }
 }
 

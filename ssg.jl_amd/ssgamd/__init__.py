@@ -50,7 +50,8 @@ EXPORTED_SYMBOLS = (
     "ssg_set_field_device", "ssg_get_field_device", "ssg_timestep", "ssg_tendency", "ssg_jacobian",
     "ssg_rfft_2d", "ssg_irfft_2d", "ssg_apply_exponential_filter", "ssg_diagnostics", "ssg_last_timing",
     "ssg_profile", "ssg_profile_read", "ssg_tuning", "ssg_ma_solve", "ssg_get_phi3d", "ssg_set_coupling",
-    "ssg_last_ma_diag", "ssg_copy_probe", "ssg_adjoint",
+    "ssg_last_ma_diag", "ssg_copy_probe", "ssg_adjoint", "ssg_ensemble_create", "ssg_ensemble_destroy",
+    "ssg_ensemble_member", "ssg_ensemble_last_error", "ssg_ensemble_timestep", "ssg_ensemble_tuning",
 )
 
 MA_BODY = 0      # as wired: b on every level, surface slope 0
@@ -151,9 +152,17 @@ def load_library(path: str | None = None):
     lib.ssg_last_ma_diag.argtypes = [vp, ctypes.POINTER(_MaDiag)]
     lib.ssg_copy_probe.argtypes = [vp, ctypes.c_int64, ctypes.c_int32]
     lib.ssg_adjoint.argtypes = [vp, ctypes.POINTER(_TimeParams), ctypes.c_int64, ctypes.c_int64, _dp, _dp]
+    lib.ssg_ensemble_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(_DomainDesc)]
+    lib.ssg_ensemble_destroy.argtypes = [vp]
+    lib.ssg_ensemble_member.argtypes = [vp, ctypes.c_int32]
+    lib.ssg_ensemble_last_error.argtypes = [vp]
+    lib.ssg_ensemble_timestep.argtypes = [vp, ctypes.POINTER(_TimeParams), _dp, ctypes.POINTER(_TimeState),
+                                          ctypes.c_int64]
+    lib.ssg_ensemble_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), _dp]
     lib.ssg_profile_read.argtypes = [vp, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), _dp]
     for n in EXPORTED_SYMBOLS:
-        getattr(lib, n).restype = ctypes.c_char_p if n == "ssg_last_error" else ctypes.c_int
+        getattr(lib, n).restype = ctypes.c_char_p if n in ("ssg_last_error", "ssg_ensemble_last_error") else ctypes.c_int
+    lib.ssg_ensemble_member.restype = vp
     if path is None:
         _lib = lib
     return lib
@@ -175,16 +184,20 @@ class Domain:
     """
 
     def __init__(self, nx: int, ny: int, nz: int = 1, Lx: float = 2 * np.pi, Ly: float = 2 * np.pi,
-                 Lz: float = 1.0, precision: int = SSG_FP64, device: int = 0):
+                 Lz: float = 1.0, precision: int = SSG_FP64, device: int = 0, _borrowed=None):
         self.lib = load_library()
         self.Nx, self.Ny, self.Nz = int(nx), int(ny), int(nz)
         self.Lx, self.Ly, self.Lz = float(Lx), float(Ly), float(Lz)
         self.precision = int(precision)
-        desc = _DomainDesc(self.Nx, self.Ny, self.Nz, self.Lx, self.Ly, self.Lz, self.precision, int(device))
-        h = ctypes.c_void_p()
-        rc = self.lib.ssg_create(ctypes.byref(h), ctypes.byref(desc))
-        if rc != SSG_OK:
-            raise SSGError(rc, "ssg_create failed (see stderr)")
+        self._owned = _borrowed is None
+        if _borrowed is not None:
+            h = ctypes.c_void_p(_borrowed)
+        else:
+            desc = _DomainDesc(self.Nx, self.Ny, self.Nz, self.Lx, self.Ly, self.Lz, self.precision, int(device))
+            h = ctypes.c_void_p()
+            rc = self.lib.ssg_create(ctypes.byref(h), ctypes.byref(desc))
+            if rc != SSG_OK:
+                raise SSGError(rc, "ssg_create failed (see stderr)")
         self._h = h
         self.nyc = self.Ny // 2 + 1
         # src/domain.jl:145-162
@@ -205,7 +218,8 @@ class Domain:
 
     def close(self):
         if getattr(self, "_h", None):
-            self.lib.ssg_destroy(self._h)
+            if self._owned:
+                self.lib.ssg_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -448,3 +462,61 @@ def timestep_adjoint(domain: Domain, params: TimeParams, step_end: int, nsteps: 
     domain.check(domain.lib.ssg_adjoint(domain.handle, ctypes.byref(p), int(step_end), int(nsteps),
                                         lam.ctypes.data_as(_dp), out.ctypes.data_as(_dp)))
     return out
+
+
+class Ensemble:
+    """Members on their own grids stepped together with one launch per kernel
+    class per RK stage (``ssg_ensemble_*``).  ``members`` holds
+    ``(nx, ny[, nz])`` tuples; ``member(i)`` is a :class:`Domain` view for
+    setting fields and running per-member solves."""
+
+    def __init__(self, members, precision: int = SSG_FP64, device: int = 0, Lx=2 * np.pi, Ly=2 * np.pi, Lz=1.0):
+        self.lib = load_library()
+        descs = (_DomainDesc * len(members))()
+        for i, m in enumerate(members):
+            nx, ny = m[0], m[1]
+            nz = m[2] if len(m) > 2 else 1
+            descs[i] = _DomainDesc(nx, ny, nz, Lx, Ly, Lz, int(precision), int(device))
+        h = ctypes.c_void_p()
+        rc = self.lib.ssg_ensemble_create(ctypes.byref(h), len(members), descs)
+        if rc != SSG_OK:
+            raise SSGError(rc, "ssg_ensemble_create failed (see stderr)")
+        self._h = h
+        self.n = len(members)
+        self.precision = int(precision)
+        self.members = [Domain(d.nx, d.ny, d.nz, Lx, Ly, Lz, precision, device,
+                               _borrowed=self.lib.ssg_ensemble_member(h, i)) for i, d in enumerate(descs)]
+        self.states = [TimeState(0.0) for _ in range(self.n)]
+
+    def member(self, i: int) -> Domain:
+        return self.members[i]
+
+    def timestep(self, params: TimeParams, dts, nsteps: int = 1):
+        p = params._c()
+        dt = np.ascontiguousarray(dts, dtype=np.float64)
+        st = (_TimeState * self.n)(*[s._c for s in self.states])
+        rc = self.lib.ssg_ensemble_timestep(self._h, ctypes.byref(p), dt.ctypes.data_as(_dp), st, int(nsteps))
+        for i in range(self.n):
+            self.states[i]._c = st[i]
+        if rc != SSG_OK:
+            msg = self.lib.ssg_ensemble_last_error(self._h)
+            raise SSGError(rc, msg.decode() if msg else "")
+
+    def tuning(self):
+        ept = ctypes.c_int32()
+        us = (ctypes.c_double * 3)()
+        self.lib.ssg_ensemble_tuning(self._h, ctypes.byref(ept), us)
+        return {"ept": ept.value, "cand_us": {4: us[0], 8: us[1], 16: us[2]}}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for m in self.members:
+                m._h = None
+            self.lib.ssg_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
