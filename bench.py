@@ -131,6 +131,44 @@ def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec):
             "warm_solve_last_relative_residual": last.relative_residual, "ma_kernels": kernels}
 
 
+def measure_ensemble(ssg, steps, warmup):
+    """Batched configuration: 64 members (32 x 128², 24 x 256², 8 x 512²),
+    ICs from seeds 0-63, per-member ε ~ U[0.05, 0.2] (SplitMix64, stream 1)
+    used in each member's body-forced Monge-Ampère solve for its frozen φₛ,
+    RK3 dt = 1e-3·128/Nx, FP32-mixed, filter every 10 steps."""
+    from ssgamd.ics import spectral_b0, uniform_keyed
+    sizes = [128] * 32 + [256] * 24 + [512] * 8
+    nzs = [16] * 32 + [16] * 24 + [32] * 8
+    ens = ssg.Ensemble([(n, n, z) for n, z in zip(sizes, nzs)], precision=ssg.SSG_FP32_MIXED)
+    eps = []
+    t0 = time.perf_counter()
+    iters = []
+    for i, n in enumerate(sizes):
+        e_i = 0.05 + 0.15 * float(uniform_keyed(i, np.array(0), np.array(0), stream=1))
+        eps.append(e_i)
+        m = ens.member(i)
+        f = ssg.allocate_fields(m)
+        f.bs = spectral_b0(n, n, seed=i)
+        diag = ssg.solve_monge_ampere(f, m, eps=e_i, tol=1e-10, maxiter=60, variant=ssg.MA_BODY)
+        iters.append(diag["iterations"])
+    setup_s = time.perf_counter() - t0
+    dts = [1e-3 * 128 / n for n in sizes]
+    params = ssg.TimeParams(1.0, scheme=ssg.RK3, filter_freq=10, filter_strength=36.0, filter_order=8,
+                            filter_cutoff=0.65)
+    ens.timestep(params, dts, warmup)
+    t0 = time.perf_counter()
+    ens.timestep(params, dts, steps)
+    wall = time.perf_counter() - t0
+    tune = ens.tuning()
+    ens.close()
+    points = sum(n * n for n in sizes)
+    return {"members": len(sizes), "steps": steps, "ensemble_steps_per_s": steps / wall,
+            "member_steps_per_s": len(sizes) * steps / wall, "grid_points": points,
+            "Mpoint_steps_per_s": points * steps / wall / 1e6, "autotune": tune,
+            "setup_ma_solves_s": setup_s, "setup_ma_iterations_mean": float(np.mean(iters)),
+            "eps_range": [min(eps), max(eps)]}
+
+
 def fft_compare(ssg, n, reps):
     """Spectral round trip at n x n: this library's filter application
     (k_rows_fwd + k_cols + k_rows_inv, HIP-event kernel times summed) against
@@ -174,6 +212,8 @@ def main():
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None)
     ap.add_argument("--coupled-steps", type=int, default=3,
                     help="also time this many coupled steps (Monge-Ampère solve per stage); 0 = skip")
+    ap.add_argument("--ensemble-steps", type=int, default=100,
+                    help="also time the 64-member batched configuration for this many steps; 0 = skip")
     ap.add_argument("--fft-compare", action="store_true",
                     help="only time the spectral round trip (rfft2 -> diagonal -> irfft2) of this library "
                          "against rocFFT through hipFFT, at --nx")
@@ -254,6 +294,9 @@ def main():
     if args.coupled_steps > 0:
         log(f"bench: coupled RK3 {nx}x{ny}x{nz}, {args.coupled_steps} steps")
         out["coupled_fp64"] = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP64)
+    if args.ensemble_steps > 0:
+        log(f"bench: ensemble 64 members, {args.ensemble_steps} steps")
+        out["ensemble_fp32"] = measure_ensemble(ssg, args.ensemble_steps, 20)
     if not args.no_cpu:
         log("bench: CPU baseline (oracle)")
         dt = args.dt
