@@ -96,14 +96,17 @@ def cpu_baseline(b0, psi, nx, ny, nz, dt, budget_s):
 def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec):
     """Coupled step (Monge-Ampère solve at every RK stage, body-forced
     variant, ε = 0.1, relative residual 1e-10): steps/s plus the Monge-Ampère
-    solve's iterations, time and per-kernel-class profile."""
+    solve's iterations, time and per-kernel-class profile, and the final
+    surface fields (b, φₛ) and surface kinetic energy for the cross-precision
+    comparison."""
     import ctypes
     d = ssg.make_domain(nx, ny, nz, precision=prec)
     f = ssg.allocate_fields(d)
     f.bs = b0
     diag = ssg.solve_monge_ampere(f, d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY)
-    cold = {"iterations": diag["iterations"], "ms": diag["solve_time"] * 1e3,
-            "relative_residual": diag["relative_residual"], "converged": diag["converged"]}
+    cold = {"iterations": diag["iterations"], "iterations_fp32": diag["iterations_fp32"],
+            "ms": diag["solve_time"] * 1e3, "relative_residual": diag["relative_residual"],
+            "converged": diag["converged"]}
     d.profile(True)
     ssg.solve_monge_ampere(f, d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY)
     names = {1: "k_cols (MA ops)", 2: "k_rows_inv", 3: "k_ma_rows", 4: "k_ma_tridiag", 0: "k_rows_fwd"}
@@ -113,6 +116,9 @@ def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec):
         if n:
             kernels[nm] = {"avg_us": 1e3 * ms / n, "launches": n, "GBps": by / max(ms, 1e-12) / 1e6}
     d.profile(False)
+    # restart from the cold state so both precisions step the same trajectory
+    f.bs = b0
+    ssg.solve_monge_ampere(f, d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY)
     ssg.set_coupling(d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY, warm=True)
     st = ssg.TimeState(0.0, f)
     params = ssg.TimeParams(dt, scheme=ssg.RK3, filter_freq=10, filter_strength=36.0, filter_order=8,
@@ -125,10 +131,33 @@ def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec):
     wall = time.perf_counter() - t0
     last = ssg._MaDiag()
     d.lib.ssg_last_ma_diag(d.handle, ctypes.byref(last))
+    fields = {"b": f.bs, "phis": f.phis, "ke": ssg.diagnostics(d)["ke_surface"]}
     d.close()
     return {"steps_per_s": steps / wall, "steps": steps, "variant": "body (as wired)", "eps": 0.1, "tol": 1e-10,
             "cold_solve": cold, "warm_solve_last_iterations": last.iterations,
-            "warm_solve_last_relative_residual": last.relative_residual, "ma_kernels": kernels}
+            "warm_solve_last_iterations_fp32": last.iterations_fp32,
+            "warm_solve_last_relative_residual": last.relative_residual, "ma_kernels": kernels}, fields
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def measure_ab2(ssg, b0, psi, nx, ny, nz, dt, steps):
+    """AB2 (the precision configuration's scheme) at the same grid: FP64 and
+    FP32-mixed steps/s and the rel-L2 of b between them after `steps`."""
+    params = ssg.TimeParams(dt, scheme=ssg.AB2_LowStorage, filter_freq=10, filter_strength=36.0, filter_order=8,
+                            filter_cutoff=0.65)
+    out = {}
+    ends = {}
+    for name, prec in (("fp64", ssg.SSG_FP64), ("fp32", ssg.SSG_FP32_MIXED)):
+        wall, gpu_ms, launches, b_end, _, _ = run_gpu(ssg, prec, b0, psi, nx, ny, nz, params, 0, steps, False)
+        out[f"{name}_steps_per_s"] = steps / wall
+        ends[name] = b_end
+    out["fp32_over_fp64"] = out["fp32_steps_per_s"] / out["fp64_steps_per_s"]
+    out["b_rel_l2_fp32_vs_fp64"] = rel_l2(ends["fp32"], ends["fp64"])
+    out["steps"] = steps
+    return out
 
 
 def measure_ensemble(ssg, steps, warmup):
@@ -212,6 +241,8 @@ def main():
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None)
     ap.add_argument("--coupled-steps", type=int, default=3,
                     help="also time this many coupled steps (Monge-Ampère solve per stage); 0 = skip")
+    ap.add_argument("--ab2-steps", type=int, default=1000,
+                    help="also time this many AB2 steps per precision; 0 = skip")
     ap.add_argument("--ensemble-steps", type=int, default=100,
                     help="also time the 64-member batched configuration for this many steps; 0 = skip")
     ap.add_argument("--fft-compare", action="store_true",
@@ -272,7 +303,8 @@ def main():
     if "fp64" in res and "fp32" in res:
         out["fp32_over_fp64"] = out["value"] / out["fp64_steps_per_s"]
         bf, b6 = res["fp32"][3], res["fp64"][3]
-        out["b_rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(bf - b6) / np.linalg.norm(b6))
+        out["b_rel_l2_fp32_vs_fp64"] = rel_l2(bf, b6)
+        out["b_variance_rel_fp32_vs_fp64"] = float(abs(np.sum(bf * bf) - np.sum(b6 * b6)) / np.sum(b6 * b6))
         out["b_rel_l2_steps"] = args.warmup + args.steps
     if prof:
         kinds = {KIND_NAMES[k]: {"ms": v[0], "launches": v[1], "bytes": v[2]} for k, v in prof.items()}
@@ -291,10 +323,21 @@ def main():
                               "GBps": v["bytes"] / max(v["ms"], 1e-12) / 1e6} for k, v in kinds.items()}
         out["step_algorithmic_GBps"] = tot_b / (tot_ms / 1e3) / 1e9 if tot_ms > 0 else None
     out["autotune"] = {k: v[5] for k, v in res.items()}
-    if args.coupled_steps > 0:
-        log(f"bench: coupled RK3 {nx}x{ny}x{nz}, {args.coupled_steps} steps")
-        out["coupled_fp64"] = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP64)
-    if args.ensemble_steps > 0:
+    if args.ab2_steps > 0 and not args.only:
+        log(f"bench: AB2 {nx}x{ny}, {args.ab2_steps} steps per precision")
+        out["ab2"] = measure_ab2(ssg, b0, psi, nx, ny, nz, args.dt, args.ab2_steps)
+    if args.coupled_steps > 0 and not args.only:
+        log(f"bench: coupled RK3 {nx}x{ny}x{nz}, {args.coupled_steps} steps per precision")
+        c64, f64 = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP64)
+        c32, f32 = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP32_MIXED)
+        out["coupled_fp64"] = c64
+        out["coupled_fp32"] = c32
+        out["coupled_fp32_over_fp64"] = c32["steps_per_s"] / c64["steps_per_s"]
+        out["coupled_fp32_errors"] = {"b_rel_l2": rel_l2(f32["b"], f64["b"]),
+                                      "phis_rel_l2": rel_l2(f32["phis"], f64["phis"]),
+                                      "ke_surface_rel": abs(f32["ke"] - f64["ke"]) / abs(f64["ke"]),
+                                      "steps": args.coupled_steps + 1}
+    if args.ensemble_steps > 0 and not args.only:
         log(f"bench: ensemble 64 members, {args.ensemble_steps} steps")
         out["ensemble_fp32"] = measure_ensemble(ssg, args.ensemble_steps, 20)
     if not args.no_cpu:

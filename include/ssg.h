@@ -117,6 +117,8 @@ typedef struct {
     double relative_residual;  /* ‖r‖₂ / ‖b + εD‖₂ over interior levels */
     double history[64];        /* relative residual per iteration (first 64) */
     double solve_ms;
+    int32_t iterations_fp32;   /* leading iterations run in FP32 arithmetic (FP32-mixed contexts) */
+    int32_t reserved;
 } ssg_ma_diag;
 
 /* make_domain + allocate_fields + TimeState (src/domain.jl:217-230,

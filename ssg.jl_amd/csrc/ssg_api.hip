@@ -1,6 +1,7 @@
 // C ABI of the SSG time-step library: context, field transfer, the step
 // driver and the spectral sub-boundary.  Declarations: include/ssg.h.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -76,6 +77,10 @@ struct ssg_ctx {
     int ma_partial_cap = 0;
     int ma_partial_used = 0;
     bool ma_have_d = false;
+    // FP32 phase of the mixed-precision solve: Φ̂ and D̂ in FP32; the three
+    // derivative planes alias ma_p (an FP64 plane holds two FP32 planes)
+    cpx<float>* ma32_phi = nullptr;
+    cpx<float>* ma32_d = nullptr;
     Grid g3{};                        // 3D plane geometry: pitch = nz * ny
     // coupled stepping (MA solve at every stage)
     bool coupled = false;
@@ -191,6 +196,19 @@ int lines_per_block(int n, int e, int rows, int nbuf = 1) {
 int prof_begin(ssg_ctx* c, int* slot);
 int prof_end(ssg_ctx* c, int slot, int kind, double bytes);
 
+// Launch on the context stream.  With a profiling slot the dispatch itself
+// timestamps the slot's event pair at kernel start and end
+// (hipExtLaunchKernelGGL), so the measured time is the kernel's own
+// duration, without the event packets of a separate record.
+template <typename K, typename... Args>
+void launch_on(ssg_ctx* c, int slot, K kernel, dim3 grid, dim3 block, size_t lds, Args... args) {
+    if (slot >= 0)
+        hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)lds, c->stream, c->ev_pool[slot],
+                              c->ev_pool[slot + 1], 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, lds, c->stream, args...);
+}
+
 template <typename R>
 PathBuffers<R>& bufs(ssg_ctx* c);
 template <>
@@ -226,7 +244,7 @@ int launch_rows_fwd(ssg_ctx* c, const S* f, const cpx<R>* w, cpx<R>* PA, cpx<R>*
                                              : n2 * sizeof(S) + plane;
         int slot, rc;
         if ((rc = prof_begin(c, &slot))) return rc;
-        hipLaunchKernelGGL((k_rows_fwd<R, S, N, E, SRC>), dim3(nblk), dim3(G * T), lds, c->stream, a);
+        launch_on(c, slot, k_rows_fwd<R, S, N, E, SRC>, dim3(nblk), dim3(G * T), lds, a);
         c->launches++;
         HIP_OK(c, hipGetLastError());
         return prof_end(c, slot, 0, bytes);
@@ -262,7 +280,7 @@ int launch_cols_args(ssg_ctx* c, ColsArgs<R> a, int nlines, int nlev) {
         const double bytes = (double)(nin + nout) * nlines * c->ny * sizeof(cpx<R>);
         int slot, rc;
         if ((rc = prof_begin(c, &slot))) return rc;
-        hipLaunchKernelGGL((k_cols<R, M, E, OP>), dim3(nblk), dim3(G * T), lds, c->stream, a);
+        launch_on(c, slot, k_cols<R, M, E, OP>, dim3(nblk), dim3(G * T), lds, a);
         c->launches++;
         HIP_OK(c, hipGetLastError());
         if (OP == OP_MA_DIFF) c->ma_partial_used = nblk;
@@ -317,16 +335,16 @@ int launch_rows_inv(ssg_ctx* c, RowsInvArgs<R> a, int rows = 0, const Grid* grid
         if (a.bstage) bytes += n2 * sizeof(double);
         int slot, rc;
         if ((rc = prof_begin(c, &slot))) return rc;
-        hipLaunchKernelGGL((k_rows_inv<R, N, E, MODE, EPI>), dim3(nblk), dim3(G * T), lds, c->stream, a);
+        launch_on(c, slot, k_rows_inv<R, N, E, MODE, EPI>, dim3(nblk), dim3(G * T), lds, a);
         c->launches++;
         HIP_OK(c, hipGetLastError());
         return prof_end(c, slot, 2, bytes);
     });
 }
 
-// Event pair around one launch when profiling is on.  `bytes` is the
-// launch's algorithmic traffic: every input read once, every output
-// written once.
+// Event pair of one launch when profiling is on (recorded by launch_on).
+// `bytes` is the launch's algorithmic traffic: every input read once,
+// every output written once.
 int prof_begin(ssg_ctx* c, int* slot) {
     *slot = -1;
     if (!c->prof) return SSG_OK;
@@ -337,13 +355,11 @@ int prof_begin(ssg_ctx* c, int* slot) {
         c->ev_pool.push_back(e);
     }
     *slot = (int)c->prof_recs.size() * 2;
-    HIP_OK(c, hipEventRecord(c->ev_pool[*slot], c->stream));
     return SSG_OK;
 }
 
 int prof_end(ssg_ctx* c, int slot, int kind, double bytes) {
     if (slot < 0) return SSG_OK;
-    HIP_OK(c, hipEventRecord(c->ev_pool[slot + 1], c->stream));
     c->prof_recs.push_back({kind, bytes, slot, slot + 1});
     return SSG_OK;
 }
@@ -708,6 +724,10 @@ int ma_alloc(ssg_ctx* c) {
     HIP_OK(c, hipMalloc(&c->ma_dprev, c3 * sizeof(cpx<double>)));
     for (int i = 0; i < 3; ++i) HIP_OK(c, hipMalloc(&c->ma_p[i], c3 * sizeof(cpx<double>)));
     HIP_OK(c, hipMalloc(&c->ma_b2, (size_t)c->ncols_full * c->ny * sizeof(cpx<double>)));
+    if (c->precision == SSG_FP32_MIXED) {
+        HIP_OK(c, hipMalloc(&c->ma32_phi, c3 * sizeof(cpx<float>)));
+        HIP_OK(c, hipMalloc(&c->ma32_d, c3 * sizeof(cpx<float>)));
+    }
     c->ma_partial_cap = c->ncols_full * c->nz + 64;
     HIP_OK(c, hipMalloc(&c->ma_partial, (size_t)c->ma_partial_cap * 2 * sizeof(double)));
     HIP_OK(c, hipHostMalloc(&c->ma_partial_host, (size_t)c->ma_partial_cap * 2 * sizeof(double), hipHostMallocDefault));
@@ -747,17 +767,21 @@ void ma_free(ssg_ctx* c) {
     (void)hipFree(c->ma_dprev);
     for (int i = 0; i < 3; ++i) (void)hipFree(c->ma_p[i]);
     (void)hipFree(c->ma_b2);
+    (void)hipFree(c->ma32_phi);
+    (void)hipFree(c->ma32_d);
     (void)hipFree(c->ma_coef);
     (void)hipFree(c->ma_partial);
     if (c->ma_partial_host) (void)hipHostFree(c->ma_partial_host);
     (void)hipFree(c->bst);
 }
 
-int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d) {
-    MaTriArgs<double> a{};
-    a.Dh = c->ma_dprev;
+// Φ̂ = L⁻¹(b̂ + ε D̂) per wavenumber (D̂ term omitted when use_d = 0)
+template <typename R, typename P>
+int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d, const cpx<R>* Dh, cpx<P>* Phi) {
+    MaTriArgs<R, P> a{};
+    a.Dh = Dh;
     a.B2 = c->ma_b2;
-    a.Phi = c->ma_phi;
+    a.Phi = Phi;
     a.coef = c->ma_coef;
     a.nz = c->nz;
     a.ncols = c->ncols_full;
@@ -768,103 +792,174 @@ int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d) {
     a.g = c->g;
     const long total = (long)c->ncols_full * c->ny;
     const int nblk = (int)((total + 127) / 128);
-    const double bytes = (double)(2 * c->nz + 1) * total * sizeof(cpx<double>);
+    const double bytes = (double)total * (c->nz * (sizeof(cpx<R>) + sizeof(cpx<P>)) + sizeof(cpx<double>));
     int slot, rc;
     if ((rc = prof_begin(c, &slot))) return rc;
     if (c->nz <= 8)
-        hipLaunchKernelGGL((k_ma_tridiag<double, 8>), dim3(nblk), dim3(128), 0, c->stream, a);
+        launch_on(c, slot, k_ma_tridiag<R, P, 8>, dim3(nblk), dim3(128), 0, a);
     else if (c->nz <= 16)
-        hipLaunchKernelGGL((k_ma_tridiag<double, 16>), dim3(nblk), dim3(128), 0, c->stream, a);
+        launch_on(c, slot, k_ma_tridiag<R, P, 16>, dim3(nblk), dim3(128), 0, a);
     else if (c->nz <= 32)
-        hipLaunchKernelGGL((k_ma_tridiag<double, 32>), dim3(nblk), dim3(128), 0, c->stream, a);
+        launch_on(c, slot, k_ma_tridiag<R, P, 32>, dim3(nblk), dim3(128), 0, a);
     else if (c->nz <= 64)
-        hipLaunchKernelGGL((k_ma_tridiag<double, 64>), dim3(nblk), dim3(128), 0, c->stream, a);
+        launch_on(c, slot, k_ma_tridiag<R, P, 64>, dim3(nblk), dim3(128), 0, a);
     else
-        hipLaunchKernelGGL((k_ma_tridiag<double, 128>), dim3(nblk), dim3(128), 0, c->stream, a);
+        launch_on(c, slot, k_ma_tridiag<R, P, 128>, dim3(nblk), dim3(128), 0, a);
     c->launches++;
     HIP_OK(c, hipGetLastError());
     return prof_end(c, slot, 4, bytes);
 }
 
-int ma_rows(ssg_ctx* c) {
-    PathBuffers<double>& D = c->d;
-    const int ept = ept_of<double>(c, 2, c->nx);
+// x-inverse of the three derivative planes, D = Φxx Φyy - Φxy², x-FFT of D
+// into planes[0]
+template <typename R>
+int ma_rows(ssg_ctx* c, cpx<R>* const (&planes)[3]) {
+    PathBuffers<R>& B = bufs<R>(c);
+    const int ept = ept_of<R>(c, 2, c->nx);
     const int rows = c->nz * c->ny;
     return with_size_ept(c->nx, ept, [&](auto tag, auto etag) -> int {
         constexpr int N = decltype(tag)::value;
         constexpr int E = decltype(etag)::value;
         constexpr int T = N / E;
-        const int G = lines_per_block<double>(N, E, rows / 2, 3);
+        const int G = lines_per_block<R>(N, E, rows / 2, 3);
         const int nblk = rows / (2 * G);
-        MaRowsArgs<double> a{c->ma_p[0], c->ma_p[1], c->ma_p[2], c->ma_p[0], c->ncols_full, G, nblk, D.twx, c->g3};
-        const size_t lds = (size_t)G * 3 * lds_line_elems(N) * sizeof(cpx<double>);
-        const double bytes = 4.0 * c->ncols_full * rows * sizeof(cpx<double>);
+        MaRowsArgs<R> a{planes[0], planes[1], planes[2], planes[0], c->ncols_full, G, nblk, B.twx, c->g3};
+        const size_t lds = (size_t)G * 3 * lds_line_elems(N) * sizeof(cpx<R>);
+        const double bytes = 4.0 * c->ncols_full * rows * sizeof(cpx<R>);
         int slot, rc;
         if ((rc = prof_begin(c, &slot))) return rc;
-        hipLaunchKernelGGL((k_ma_rows<double, N, E>), dim3(nblk), dim3(G * T), lds, c->stream, a);
+        launch_on(c, slot, k_ma_rows<R, N, E>, dim3(nblk), dim3(G * T), lds, a);
         c->launches++;
         HIP_OK(c, hipGetLastError());
         return prof_end(c, slot, 3, bytes);
     });
 }
 
+template <typename A, typename B>
+int cpx_convert(ssg_ctx* c, const cpx<A>* in, cpx<B>* out, size_t n) {
+    hipLaunchKernelGGL((k_cpx_convert<A, B>), dim3(grid1d((long)n)), dim3(256), 0, c->stream, in, out, (long)n);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    return SSG_OK;
+}
+
+// Residual evaluation of the iterate Φ̂ = L⁻¹(b̂ + ε D̂_prev): D̂ of Φ into
+// `dh` (which holds D̂_prev on entry) and the residual ε(D̂_prev - D̂) on
+// interior levels, reduced over blocks and read back.  `rabs` is
+// norm_field(r) (src/poisson.jl:986-995) via Parseval, `rrel` is
+// ‖r‖ / ‖b + εD‖.
+template <typename R>
+int ma_eval(ssg_ctx* c, const ssg_ma_params* p, const cpx<R>* phi, cpx<R>* const (&planes)[3], cpx<R>* dh,
+            double* rabs, double* rrel) {
+    const double scale = 1.0 / ((double)c->nx * c->ny);
+    const int lines = c->ncols_full * c->nz;
+    ColsArgs<R> a{};
+    a.inA = phi;
+    a.outA = planes[0];
+    a.outB = planes[1];
+    a.outC = planes[2];
+    a.scale = (R)scale;
+    int rc;
+    if ((rc = launch_cols_args<R, OP_MA_DERIV>(c, a, lines, c->nz))) return rc;
+    if ((rc = ma_rows<R>(c, planes))) return rc;
+    ColsArgs<R> b{};
+    b.inA = planes[0];
+    b.outA = dh;
+    b.B2 = c->ma_b2;
+    b.eps = p->eps;
+    b.body = p->variant == SSG_MA_BODY ? 1 : 0;
+    b.partial = c->ma_partial;
+    if ((rc = launch_cols_args<R, OP_MA_DIFF>(c, b, lines, c->nz))) return rc;
+    const int nb = c->ma_partial_used;
+    HIP_OK(c, hipMemcpyAsync(c->ma_partial_host, c->ma_partial, (size_t)nb * 2 * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    double r2 = 0, h2 = 0;
+    for (int i = 0; i < nb; ++i) {
+        r2 += c->ma_partial_host[2 * i];
+        h2 += c->ma_partial_host[2 * i + 1];
+    }
+    // Parseval: sum of r² over the grid = (1/(nx ny)) sum of |r̂|² per level
+    *rabs = std::sqrt(r2 * scale);
+    *rrel = h2 > 0 ? std::sqrt(r2 / h2) : std::sqrt(r2 * scale);
+    return SSG_OK;
+}
+
+// The FP32 phase stops once its residual estimate reaches this level (or
+// stops falling); below it FP32 rounding of D̂ dominates the residual.
+constexpr double kMaFp32Floor = 1e-6;
+
 // Picard / defect-correction iteration Φ̂ <- L⁻¹(b̂ + ε D̂(Φ)); the residual
 // of iterate n is ε(D̂ⁿ⁻¹ - D̂ⁿ) on interior levels, so every iteration
 // reports the reference residual of the Φ it holds.
+//
+// Mixed precision (SSG_FP32_MIXED contexts): the iteration first runs with
+// FP32 transforms, products and D̂ (the per-wavenumber solve stays FP64
+// arithmetic, rounded to FP32 on output).  The switch solves
+// Φ̂ = L⁻¹(b̂ + ε D̂₃₂) in FP64 and carries D̂₃₂ over as D̂_prev, so the first
+// FP64 evaluation yields the exact FP64 residual of that Φ; FP64 iterations
+// then continue to `tol`.  Convergence is only ever declared on an FP64
+// residual.
 int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
-    const double scale = 1.0 / ((double)c->nx * c->ny);
-    const int lines = c->ncols_full * c->nz;
-    int use_d = 1;
-    if (!p->warm || !c->ma_have_d) {
-        HIP_OK(c, hipMemsetAsync(c->ma_dprev, 0, (size_t)lines * c->ny * sizeof(cpx<double>), c->stream));
-        use_d = 0;
-    }
-    int rc = ma_tridiag(c, p, use_d);
-    if (rc) return rc;
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    const bool warm = p->warm && c->ma_have_d;
     memset(d, 0, sizeof(*d));
+    int it = 0, rc;
     double best = 1e300;
-    for (int it = 0;; ++it) {
-        ColsArgs<double> a{};
-        a.inA = c->ma_phi;
-        a.outA = c->ma_p[0];
-        a.outB = c->ma_p[1];
-        a.outC = c->ma_p[2];
-        a.scale = scale;
-        if ((rc = launch_cols_args<double, OP_MA_DERIV>(c, a, lines, c->nz))) return rc;
-        if ((rc = ma_rows(c))) return rc;
-        ColsArgs<double> b{};
-        b.inA = c->ma_p[0];
-        b.outA = c->ma_dprev;
-        b.B2 = c->ma_b2;
-        b.eps = p->eps;
-        b.body = p->variant == SSG_MA_BODY ? 1 : 0;
-        b.partial = c->ma_partial;
-        if ((rc = launch_cols_args<double, OP_MA_DIFF>(c, b, lines, c->nz))) return rc;
-        const int nb = c->ma_partial_used;
-        HIP_OK(c, hipMemcpyAsync(c->ma_partial_host, c->ma_partial, (size_t)nb * 2 * sizeof(double),
-                                 hipMemcpyDeviceToHost, c->stream));
-        HIP_OK(c, hipStreamSynchronize(c->stream));
-        double r2 = 0, h2 = 0;
-        for (int i = 0; i < nb; ++i) {
-            r2 += c->ma_partial_host[2 * i];
-            h2 += c->ma_partial_host[2 * i + 1];
-        }
-        // Parseval: sum of r² over the grid = (1/(nx ny)) sum of |r̂|² per level
-        d->final_residual = std::sqrt(r2 * scale);
-        d->relative_residual = h2 > 0 ? std::sqrt(r2 / h2) : std::sqrt(r2 * scale);
-        if (it < 64) d->history[it] = d->relative_residual;
+    double ra = 0, rr = 0;
+    // record one evaluation; returns true when the iteration must stop
+    auto record = [&](bool fp64) -> bool {
+        d->final_residual = ra;
+        d->relative_residual = rr;
+        if (it < 64) d->history[it] = rr;
         d->iterations = it;
-        if (!std::isfinite(d->relative_residual)) break;
+        ++it;
+        if (!std::isfinite(rr)) return true;
         // divergence (loss of ellipticity): stop once the residual has grown
         // a hundredfold over the best iterate
-        best = std::min(best, d->relative_residual);
-        if (d->relative_residual > 100.0 * best) break;
-        if (d->relative_residual <= p->tol) {
+        best = std::min(best, rr);
+        if (rr > 100.0 * best) return true;
+        if (fp64 && rr <= p->tol) {
             d->converged = 1;
-            break;
+            return true;
         }
-        if (it >= p->maxiter) break;
-        if ((rc = ma_tridiag(c, p, 1))) return rc;
+        return it > p->maxiter;
+    };
+    if (c->precision == SSG_FP32_MIXED) {
+        cpx<float>* const pl[3] = {(cpx<float>*)c->ma_p[0], (cpx<float>*)c->ma_p[1], (cpx<float>*)c->ma_p[2]};
+        if (warm) {
+            if ((rc = cpx_convert<double, float>(c, c->ma_dprev, c->ma32_d, c3))) return rc;
+        } else {
+            HIP_OK(c, hipMemsetAsync(c->ma32_d, 0, c3 * sizeof(cpx<float>), c->stream));
+        }
+        if ((rc = ma_tridiag<float, float>(c, p, warm ? 1 : 0, c->ma32_d, c->ma32_phi))) return rc;
+        double prev = 1e300;
+        bool stop = false;
+        for (;;) {
+            if ((rc = ma_eval<float>(c, p, c->ma32_phi, pl, c->ma32_d, &ra, &rr))) return rc;
+            d->iterations_fp32++;
+            stop = record(false);
+            if (stop || rr <= std::max(p->tol, kMaFp32Floor) || rr > 0.7 * prev) break;
+            prev = rr;
+            if ((rc = ma_tridiag<float, float>(c, p, 1, c->ma32_d, c->ma32_phi))) return rc;
+        }
+        // switch to FP64; a maxiter stop still gets one FP64 evaluation, so
+        // the reported residual is always an FP64 one
+        if ((rc = cpx_convert<float, double>(c, c->ma32_d, c->ma_dprev, c3))) return rc;
+        if ((rc = ma_tridiag<float, double>(c, p, 1, c->ma32_d, c->ma_phi))) return rc;
+        if (stop && (!std::isfinite(rr) || rr > 100.0 * best)) {
+            c->ma_have_d = false;
+            return SSG_OK;
+        }
+    } else {
+        if (!warm) HIP_OK(c, hipMemsetAsync(c->ma_dprev, 0, c3 * sizeof(cpx<double>), c->stream));
+        if ((rc = ma_tridiag<double, double>(c, p, warm ? 1 : 0, c->ma_dprev, c->ma_phi))) return rc;
+    }
+    cpx<double>* const pl[3] = {c->ma_p[0], c->ma_p[1], c->ma_p[2]};
+    for (;;) {
+        if ((rc = ma_eval<double>(c, p, c->ma_phi, pl, c->ma_dprev, &ra, &rr))) return rc;
+        if (record(true)) break;
+        if ((rc = ma_tridiag<double, double>(c, p, 1, c->ma_dprev, c->ma_phi))) return rc;
     }
     c->ma_have_d = true;
     return SSG_OK;

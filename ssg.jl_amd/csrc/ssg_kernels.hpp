@@ -815,6 +815,13 @@ __global__ void k_convert(const A* __restrict__ in, B* __restrict__ out, long n)
         out[q] = (B)in[q];
 }
 
+// complex array precision change (FP32 <-> FP64)
+template <typename A, typename B>
+__global__ void k_cpx_convert(const cpx<A>* __restrict__ in, cpx<B>* __restrict__ out, long n) {
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        out[q] = cpx<B>{(B)in[q].x, (B)in[q].y};
+}
+
 template <typename R>
 __global__ void k_w_convert(const double2* __restrict__ in, cpx<R>* __restrict__ out, long n) {
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)

@@ -118,11 +118,13 @@ __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
 //   c[3k+0..2] = (alpha, beta, gamma) of d2dz2 at interior level k
 //   c[3nz+0..2] = (A, B, C) bottom row  A Φ0 + B Φ1 + C Φ2 = 0
 //   c[3nz+3..5] = (a, b, c) top row     a Φn-1 + b Φn-2 + c Φn-3 = g
-template <typename R>
+// R: precision of the D̂ input, P: precision of the Φ̂ output; the solve
+// itself is FP64 arithmetic in every case
+template <typename R, typename P = double>
 struct MaTriArgs {
     const cpx<R>* Dh;           // [kx][lev][ky]
     const cpx<double>* B2;      // [kx][ky] forcing spectrum
-    cpx<double>* Phi;           // [kx][lev][ky]
+    cpx<P>* Phi;                // [kx][lev][ky]
     const double* coef;
     int nz;
     int ncols;
@@ -133,8 +135,8 @@ struct MaTriArgs {
     Grid g;
 };
 
-template <typename R, int NZMAX>
-__global__ __launch_bounds__(128) void k_ma_tridiag(MaTriArgs<R> a) {
+template <typename R, typename P, int NZMAX>
+__global__ __launch_bounds__(128) void k_ma_tridiag(MaTriArgs<R, P> a) {
     const long total = (long)a.ncols * a.g.ny;
     const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (q >= total) return;
@@ -201,12 +203,9 @@ __global__ __launch_bounds__(128) void k_ma_tridiag(MaTriArgs<R> a) {
     for (int k = NZMAX - 1; k >= 0; --k) {
         if (k <= nz - 1) {
             x = {dp[k].x - cp[k] * x.x, dp[k].y - cp[k] * x.y};
-            a.Phi[base + (size_t)k * ny] = x;
+            a.Phi[base + (size_t)k * ny] = cpx<P>{(P)x.x, (P)x.y};
         }
     }
 }
-
-// residual of the linear part for a given Φ̂ (used for diagnostics of the
-// first iterate): nothing else needed.
 
 }  // namespace ssg

@@ -106,13 +106,14 @@ class _MaParams(ctypes.Structure):
 class _MaDiag(ctypes.Structure):
     _fields_ = [("converged", ctypes.c_int32), ("iterations", ctypes.c_int32), ("final_residual", ctypes.c_double),
                 ("relative_residual", ctypes.c_double), ("history", ctypes.c_double * 64),
-                ("solve_ms", ctypes.c_double)]
+                ("solve_ms", ctypes.c_double), ("iterations_fp32", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
     def as_dict(self):
         n = min(self.iterations + 1, 64)
         return {"converged": bool(self.converged), "iterations": self.iterations,
                 "final_residual": self.final_residual, "relative_residual": self.relative_residual,
-                "convergence_history": list(self.history[:n]), "solve_time": self.solve_ms / 1e3}
+                "convergence_history": list(self.history[:n]), "solve_time": self.solve_ms / 1e3,
+                "iterations_fp32": self.iterations_fp32}
 
 
 _lib = None

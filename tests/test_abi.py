@@ -52,6 +52,7 @@ def test_struct_layouts_match_c():
     assert ctypes.sizeof(ssgamd._TimeParams) == 48
     assert ssgamd._TimeParams.filter_strength.offset == 16
     assert ctypes.sizeof(ssgamd._TimeState) == 24
+    assert ctypes.sizeof(ssgamd._MaDiag) == 552
 
 
 def test_ics_are_deterministic_and_normalised():
