@@ -182,6 +182,23 @@ int ssg_set_coupling(ssg_ctx* ctx, const ssg_ma_params* params);
 int ssg_adjoint(ssg_ctx* ctx, const ssg_time_params* params, int64_t step_end, int64_t nsteps, const double* lam_in,
                 double* lam_out);
 
+/* Vector-Jacobian product of the Monge-Ampère solve at the solution of the
+   last ssg_ma_solve (same eps/variant in params): bbar = (∂φₛ/∂bₛ)ᵀ phibar,
+   the transposed linearised solve of solve_monge_ampere_fields!
+   (src/poisson.jl:1068-1090; residual operator src/poisson.jl:143-364).
+   phibar, bbar: host, nx*ny FP64.  diag (optional) reports the transposed
+   iteration; SSG_ERR_NOT_CONVERGED if it misses params->tol. */
+int ssg_ma_adjoint(ssg_ctx* ctx, const ssg_ma_params* params, const double* phibar, double* bbar, ssg_ma_diag* diag);
+/* Discrete adjoint of `nsteps` coupled RK3 steps (Monge-Ampère solve at
+   every stage, params `ma`), starting from the context's current b at step
+   number step_start: lam_out = (∂b(step_start+nsteps)/∂b(step_start))ᵀ lam_in.
+   The trajectory is recomputed in FP64 from checkpoints taken every
+   `checkpoint_every` steps; the context's b is left at the start state.
+   The step is timestep! (src/timestep.jl:195-242, :267-285) with φₛ
+   re-solved per stage. */
+int ssg_adjoint_coupled(ssg_ctx* ctx, const ssg_time_params* params, const ssg_ma_params* ma, int64_t step_start,
+                        int64_t nsteps, int32_t checkpoint_every, const double* lam_in, double* lam_out);
+
 /* diagnostics of the most recent Monge-Ampère solve */
 int ssg_last_ma_diag(ssg_ctx* ctx, ssg_ma_diag* diag);
 

@@ -90,6 +90,21 @@ struct ssg_ctx {
     double* adj_lam = nullptr;        // adjoint state
     double* adj_acc = nullptr;
     double* adj_s = nullptr;
+    // coupled-step adjoint (allocated on first use)
+    bool adjc_ready = false;
+    double* adjc_pd[3] = {nullptr, nullptr, nullptr};        // Φxx, Φyy, Φxy physical, [lev][y][x]
+    cpx<double>* adjc_fbar = nullptr;                        // F̄ [kx][row][ky]
+    cpx<double>* adjc_q = nullptr;                           // D'ᵀλ of the current iterate
+    cpx<double>* adjc_lam = nullptr;                         // λ planes (y physical)
+    cpx<double>* adjc_phibar = nullptr;                      // F(φ̄) [kx][ky]
+    cpx<double>* adjc_bbar = nullptr;                        // F(b̄) [kx][ky]
+    cpx<double>* adjc_wbar = nullptr;                        // (ū, v̄)
+    double* adjc_g = nullptr;
+    double* adjc_k1 = nullptr;
+    double* adjc_tmp = nullptr;
+    double* adjc_bst[3] = {nullptr, nullptr, nullptr};       // stage buoyancies of the step being reversed
+    cpx<double>* adjc_phis[3] = {nullptr, nullptr, nullptr}; // their Monge-Ampère solutions
+    ssg_ma_diag adjc_diag{};
     struct GraphCache {
         hipGraphExec_t exec = nullptr;
         int len = 0;
@@ -234,13 +249,14 @@ int launch_rows_fwd(ssg_ctx* c, const S* f, const cpx<R>* w, cpx<R>* PA, cpx<R>*
         constexpr int N = decltype(tag)::value;
         constexpr int E = decltype(etag)::value;
         constexpr int T = N / E;
-        const int rows_per_line = SRC == SRC_FLUX ? 1 : 2;
+        const int rows_per_line = SRC == SRC_PAIR ? 2 : 1;
         const int G = lines_per_block<R>(N, E, c->ny / rows_per_line);
         const int nblk = c->ny / (rows_per_line * G);
         RowsFwdArgs<R, S> a{f, w, PA, PB, ncols, G, nblk, B.twx, c->g};
         const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<R>);
         const double n2 = (double)c->nx * c->ny, plane = (double)ncols * c->ny * sizeof(cpx<R>);
         const double bytes = SRC == SRC_FLUX ? n2 * (sizeof(double) + sizeof(cpx<R>)) + 2 * plane
+                             : SRC == SRC_W  ? n2 * sizeof(cpx<R>) + 2 * plane
                                              : n2 * sizeof(S) + plane;
         int slot, rc;
         if ((rc = prof_begin(c, &slot))) return rc;
@@ -272,10 +288,12 @@ int launch_cols_args(ssg_ctx* c, ColsArgs<R> a, int nlines, int nlev) {
         a.twy = B.twy;
         a.g = c->g;
         a.nlev = nlev;
-        if (OP == OP_MA_DIFF && nblk > c->ma_partial_cap)
+        if ((OP == OP_MA_DIFF || OP == OP_MA_ADJ) && nblk > c->ma_partial_cap)
             return set_err(c, SSG_ERR_ARG, "reduction buffer too small (%d blocks)", nblk);
         const size_t lds = (size_t)G * le;
-        const int nin = (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC || OP == OP_MA_DIFF) ? 2 : 1;
+        const int nin = OP == OP_MA_ADJ ? 3
+                        : (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC || OP == OP_MA_DIFF || OP == OP_PHIBAR) ? 2
+                                                                                                                  : 1;
         const int nout = (OP == OP_VEL || OP == OP_VEL_SPEC) ? 2 : (OP == OP_MA_DERIV ? 3 : 1);
         const double bytes = (double)(nin + nout) * nlines * c->ny * sizeof(cpx<R>);
         int slot, rc;
@@ -283,7 +301,7 @@ int launch_cols_args(ssg_ctx* c, ColsArgs<R> a, int nlines, int nlev) {
         launch_on(c, slot, k_cols<R, M, E, OP>, dim3(nblk), dim3(G * T), lds, a);
         c->launches++;
         HIP_OK(c, hipGetLastError());
-        if (OP == OP_MA_DIFF) c->ma_partial_used = nblk;
+        if (OP == OP_MA_DIFF || OP == OP_MA_ADJ) c->ma_partial_used = nblk;
         return prof_end(c, slot, 1, bytes);
     });
 }
@@ -823,7 +841,8 @@ int ma_rows(ssg_ctx* c, cpx<R>* const (&planes)[3]) {
         constexpr int T = N / E;
         const int G = lines_per_block<R>(N, E, rows / 2, 3);
         const int nblk = rows / (2 * G);
-        MaRowsArgs<R> a{planes[0], planes[1], planes[2], planes[0], c->ncols_full, G, nblk, B.twx, c->g3};
+        MaRowsArgs<R> a{planes[0], planes[1], planes[2], planes[0], {nullptr, nullptr, nullptr}, c->ncols_full, G, nblk,
+                        B.twx, c->g3};
         const size_t lds = (size_t)G * 3 * lds_line_elems(N) * sizeof(cpx<R>);
         const double bytes = 4.0 * c->ncols_full * rows * sizeof(cpx<R>);
         int slot, rc;
@@ -1082,6 +1101,381 @@ int adjoint_steps(ssg_ctx* c, const ssg_time_params* p, int64_t step_end, int64_
     return SSG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Discrete adjoint of the coupled step (Monge-Ampère solve at every stage),
+// FP64.  Stage s maps b_s -> Φ_s = MA(b_s) -> w_s = (-Dy φ_s, Dx φ_s) with
+// φ_s the top level -> k_s = T(w_s, b_s).  Its transpose, for an incoming
+// k̄:
+//   b̄_s = T_wᵀ k̄                      (frozen-velocity part)
+//       + Bᵀ Gᵀ (I - ε Nᵀ Gᵀ)⁻¹ φ̄     (through the linearised MA solve)
+//   φ̄ = Dy ū - Dx v̄,  (ū, v̄) = b_s (Dx M k̄, Dy M k̄)
+// where G is the per-wavenumber solve of k_ma_tridiag, N = D'(Φ_s) the
+// linearised D operator on interior rows and B places b on the right-hand
+// side (every interior row, or the top Neumann row for the surface
+// variant).  The bracket is solved by the transposed Picard iteration
+// F̄ <- Gᵀ(φ̄ + ε Nᵀ F̄), whose residual is ε Nᵀ(F̄ⁿ - F̄ⁿ⁻¹) -- the mirror
+// of the forward iteration.  The forward trajectory is recomputed from
+// checkpoints.
+// ---------------------------------------------------------------------------
+void adjc_free(ssg_ctx* c) {
+    for (int i = 0; i < 3; ++i) {
+        (void)hipFree(c->adjc_pd[i]);
+        (void)hipFree(c->adjc_bst[i]);
+        (void)hipFree(c->adjc_phis[i]);
+    }
+    (void)hipFree(c->adjc_fbar);
+    (void)hipFree(c->adjc_q);
+    (void)hipFree(c->adjc_lam);
+    (void)hipFree(c->adjc_phibar);
+    (void)hipFree(c->adjc_bbar);
+    (void)hipFree(c->adjc_wbar);
+    (void)hipFree(c->adjc_g);
+    (void)hipFree(c->adjc_k1);
+    (void)hipFree(c->adjc_tmp);
+}
+
+int adjc_alloc(ssg_ctx* c) {
+    int rc = ma_alloc(c);
+    if (rc) return rc;
+    if ((rc = adj_alloc(c))) return rc;
+    if (c->adjc_ready) return SSG_OK;
+    const size_t n2 = (size_t)c->nx * c->ny, r3 = n2 * c->nz;
+    const size_t c2 = (size_t)c->ncols_full * c->ny, c3 = c2 * c->nz;
+    for (int i = 0; i < 3; ++i) {
+        HIP_OK(c, hipMalloc(&c->adjc_pd[i], r3 * sizeof(double)));
+        HIP_OK(c, hipMalloc(&c->adjc_bst[i], n2 * sizeof(double)));
+        HIP_OK(c, hipMalloc(&c->adjc_phis[i], c3 * sizeof(cpx<double>)));
+    }
+    HIP_OK(c, hipMalloc(&c->adjc_fbar, c3 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->adjc_q, c3 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->adjc_lam, c3 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->adjc_phibar, c2 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->adjc_bbar, c2 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->adjc_wbar, n2 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->adjc_g, n2 * sizeof(double)));
+    HIP_OK(c, hipMalloc(&c->adjc_k1, n2 * sizeof(double)));
+    HIP_OK(c, hipMalloc(&c->adjc_tmp, n2 * sizeof(double)));
+    c->adjc_ready = true;
+    return SSG_OK;
+}
+
+int lincomb(ssg_ctx* c, double* out, double ca, const double* x, double cb, const double* y) {
+    const long n = (long)c->nx * c->ny;
+    hipLaunchKernelGGL(k_lincomb, dim3(grid1d(n)), dim3(256), 0, c->stream, out, ca, x, cb, y, n);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    return SSG_OK;
+}
+
+// inverse 2D transform of a [kx][ky] spectrum (unnormalized forward
+// convention) into a real field
+int irfft2_device(ssg_ctx* c, cpx<double>* spec, double* out) {
+    PathBuffers<double>& D = c->d;
+    const double scale = 1.0 / ((double)c->nx * c->ny);
+    int rc = launch_cols<double, OP_INV>(c, spec, nullptr, D.Tp, nullptr, c->ncols_full, scale);
+    if (rc) return rc;
+    RowsInvArgs<double> f{};
+    f.P0 = D.Tp;
+    f.ncols_in = c->ncols_full;
+    f.out_r = out;
+    return launch_rows_inv<double, MODE_PAIR, EPI_STORE>(c, f);
+}
+
+// Φxx, Φyy, Φxy of the current ma_phi in physical space (adjc_pd)
+int ma_phys_derivs(ssg_ctx* c) {
+    PathBuffers<double>& D = c->d;
+    const int lines = c->ncols_full * c->nz;
+    ColsArgs<double> a{};
+    a.inA = c->ma_phi;
+    a.outA = c->ma_p[0];
+    a.outB = c->ma_p[1];
+    a.outC = c->ma_p[2];
+    a.scale = 1.0 / ((double)c->nx * c->ny);
+    int rc = launch_cols_args<double, OP_MA_DERIV>(c, a, lines, c->nz);
+    if (rc) return rc;
+    const int ept = ept_of<double>(c, 2, c->nx);
+    const int rows = c->nz * c->ny;
+    return with_size_ept(c->nx, ept, [&](auto tag, auto etag) -> int {
+        constexpr int N = decltype(tag)::value;
+        constexpr int E = decltype(etag)::value;
+        constexpr int T = N / E;
+        const int G = lines_per_block<double>(N, E, rows / 2, 3);
+        const int nblk = rows / (2 * G);
+        MaRowsArgs<double> m{c->ma_p[0], c->ma_p[1], c->ma_p[2], nullptr,
+                             {c->adjc_pd[0], c->adjc_pd[1], c->adjc_pd[2]}, c->ncols_full, G, nblk, D.twx, c->g3};
+        const size_t lds = (size_t)G * 3 * lds_line_elems(N) * sizeof(cpx<double>);
+        hipLaunchKernelGGL((k_ma_rows<double, N, E, MA_ROWS_PHYS>), dim3(nblk), dim3(G * T), lds, c->stream, m);
+        c->launches++;
+        HIP_OK(c, hipGetLastError());
+        return SSG_OK;
+    });
+}
+
+int ma_adj_rows(ssg_ctx* c) {
+    PathBuffers<double>& D = c->d;
+    const int ept = ept_of<double>(c, 2, c->nx);
+    const int rows = c->nz * c->ny;
+    return with_size_ept(c->nx, ept, [&](auto tag, auto etag) -> int {
+        constexpr int N = decltype(tag)::value;
+        constexpr int E = decltype(etag)::value;
+        constexpr int T = N / E;
+        const int G = lines_per_block<double>(N, E, rows / 2, 3);
+        const int nblk = rows / (2 * G);
+        MaAdjRowsArgs<double> a{c->adjc_lam, c->adjc_pd[0], c->adjc_pd[1], c->adjc_pd[2],
+                                {c->ma_p[0], c->ma_p[1], c->ma_p[2]}, c->ncols_full, G, nblk, D.twx, c->g3};
+        const size_t lds = (size_t)G * 3 * lds_line_elems(N) * sizeof(cpx<double>);
+        hipLaunchKernelGGL((k_ma_adj_rows<double, N, E>), dim3(nblk), dim3(G * T), lds, c->stream, a);
+        c->launches++;
+        HIP_OK(c, hipGetLastError());
+        return SSG_OK;
+    });
+}
+
+int ma_tridiag_adj(ssg_ctx* c, const ssg_ma_params* p, int use_q) {
+    MaTriAdjArgs a{};
+    a.q = c->adjc_q;
+    a.phibar = c->adjc_phibar;
+    a.Fbar = c->adjc_fbar;
+    a.bbar = c->adjc_bbar;
+    a.coef = c->ma_coef;
+    a.nz = c->nz;
+    a.ncols = c->ncols_full;
+    a.eps = p->eps;
+    a.body = p->variant == SSG_MA_BODY ? 1 : 0;
+    a.top = p->variant == SSG_MA_SURFACE ? 1 : 0;
+    a.use_q = use_q;
+    a.g = c->g;
+    const long total = (long)c->ncols_full * c->ny;
+    const int nblk = (int)((total + 127) / 128);
+    if (c->nz <= 8)
+        hipLaunchKernelGGL((k_ma_tridiag_adj<8>), dim3(nblk), dim3(128), 0, c->stream, a);
+    else if (c->nz <= 16)
+        hipLaunchKernelGGL((k_ma_tridiag_adj<16>), dim3(nblk), dim3(128), 0, c->stream, a);
+    else if (c->nz <= 32)
+        hipLaunchKernelGGL((k_ma_tridiag_adj<32>), dim3(nblk), dim3(128), 0, c->stream, a);
+    else if (c->nz <= 64)
+        hipLaunchKernelGGL((k_ma_tridiag_adj<64>), dim3(nblk), dim3(128), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_ma_tridiag_adj<128>), dim3(nblk), dim3(128), 0, c->stream, a);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    return SSG_OK;
+}
+
+// Transposed linearised Monge-Ampère solve at the current ma_phi (whose
+// physical derivative fields are in adjc_pd): adjc_phibar = F(φ̄) in,
+// adjc_bbar = F(b̄) out.
+int ma_adjoint_solve(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
+    const double scale = 1.0 / ((double)c->nx * c->ny);
+    const int lines = c->ncols_full * c->nz;
+    const size_t c3 = (size_t)lines * c->ny;
+    memset(d, 0, sizeof(*d));
+    HIP_OK(c, hipMemsetAsync(c->adjc_q, 0, c3 * sizeof(cpx<double>), c->stream));
+    int rc = ma_tridiag_adj(c, p, 0);
+    if (rc) return rc;
+    double best = 1e300;
+    for (int it = 0;; ++it) {
+        ColsArgs<double> a{};
+        a.inA = c->adjc_fbar;
+        a.outA = c->adjc_lam;
+        a.scale = scale;
+        if ((rc = launch_cols_args<double, OP_INV>(c, a, lines, c->nz))) return rc;
+        if ((rc = ma_adj_rows(c))) return rc;
+        ColsArgs<double> b{};
+        b.inA = c->ma_p[0];
+        b.inB = c->ma_p[1];
+        b.inC = c->ma_p[2];
+        b.outA = c->adjc_q;
+        b.B2 = c->adjc_phibar;
+        b.eps = p->eps;
+        b.partial = c->ma_partial;
+        if ((rc = launch_cols_args<double, OP_MA_ADJ>(c, b, lines, c->nz))) return rc;
+        const int nb = c->ma_partial_used;
+        HIP_OK(c, hipMemcpyAsync(c->ma_partial_host, c->ma_partial, (size_t)nb * 2 * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(c, hipStreamSynchronize(c->stream));
+        double r2 = 0, h2 = 0;
+        for (int i = 0; i < nb; ++i) {
+            r2 += c->ma_partial_host[2 * i];
+            h2 += c->ma_partial_host[2 * i + 1];
+        }
+        d->final_residual = std::sqrt(r2 * scale);
+        d->relative_residual = h2 > 0 ? std::sqrt(r2 / h2) : 0.0;
+        if (it < 64) d->history[it] = d->relative_residual;
+        d->iterations = it;
+        if (!std::isfinite(d->relative_residual)) break;
+        best = std::min(best, d->relative_residual);
+        if (d->relative_residual > 100.0 * best) break;
+        if (d->relative_residual <= p->tol) {
+            d->converged = 1;
+            break;
+        }
+        if (it >= p->maxiter) break;
+        if ((rc = ma_tridiag_adj(c, p, 1))) return rc;
+    }
+    c->adjc_diag = *d;
+    if (!d->converged)
+        return set_err(c, SSG_ERR_NOT_CONVERGED, "adjoint Monge-Ampere solve did not converge (rel. residual %.3e)",
+                       d->relative_residual);
+    return SSG_OK;
+}
+
+// One coupled RK3 step in FP64 from c->b, recording the stage buoyancies
+// and their Monge-Ampère solutions; c->b ends at the next step's state.
+int coupled_step_record(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* mp, int64_t step_after) {
+    const double dt = p->dt;
+    const size_t n2 = (size_t)c->nx * c->ny;
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    ssg_ma_diag d{};
+    int rc;
+    HIP_OK(c, hipMemcpyAsync(c->adjc_bst[0], c->b, n2 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    const int epi_next[3] = {EPI_RK1, EPI_RK2, EPI_RK3};
+    for (int st = 0; st < 3; ++st) {
+        if ((rc = ma_solve_dev(c, c->adjc_bst[st], mp, &d))) return rc;
+        if (!d.converged)
+            return set_err(c, SSG_ERR_NOT_CONVERGED, "Monge-Ampere solve did not converge (rel. residual %.3e)",
+                           d.relative_residual);
+        HIP_OK(c, hipMemcpyAsync(c->adjc_phis[st], c->ma_phi, c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice,
+                                 c->stream));
+        if ((rc = flux_forward<double>(c, c->ncols_tend, c->adjc_bst[st]))) return rc;
+        if ((rc = tendency_cols<double>(c, 0))) return rc;
+        if (epi_next[st] == EPI_RK1)
+            rc = stage_inverse<double, EPI_RK1>(c, dt / 3.0, 0, 0, c->adjc_bst[1]);
+        else if (epi_next[st] == EPI_RK2)
+            rc = stage_inverse<double, EPI_RK2>(c, 2.0 * dt / 3.0, 0, 0, c->adjc_bst[2]);
+        else
+            rc = stage_inverse<double, EPI_RK3>(c, dt / 4.0, 0, 0);
+        if (rc) return rc;
+    }
+    if (p->filter_freq > 0 && (step_after % p->filter_freq) == 0)
+        return filter_b<double>(c, p->filter_strength, p->filter_order, p->filter_cutoff, 0);
+    return SSG_OK;
+}
+
+// b̄ of one stage into adjc_g, for the incoming tendency adjoint `kbar`
+int adjc_stage(ssg_ctx* c, const ssg_ma_params* mp, const double* kbar, int st) {
+    PathBuffers<double>& D = c->d;
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    const double scale = 1.0 / ((double)c->nx * c->ny);
+    // the stage's Monge-Ampère solution, its velocities and derivative fields
+    HIP_OK(c, hipMemcpyAsync(c->ma_phi, c->adjc_phis[st], c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice,
+                             c->stream));
+    int rc = ma_surface_outputs(c);
+    if (rc) return rc;
+    if ((rc = ma_phys_derivs(c))) return rc;
+    // frozen-velocity transpose and the velocity adjoint (ū, v̄)
+    if ((rc = launch_rows_fwd<double, double, SRC_PAIR>(c, kbar, nullptr, D.Tp, nullptr, c->ncols_tend))) return rc;
+    ColsArgs<double> a{};
+    a.inA = D.Tp;
+    a.outA = D.Fx;
+    a.outB = D.Fy;
+    a.scale = scale;
+    if ((rc = launch_cols_args<double, OP_ADJ>(c, a, c->ncols_tend, 1))) return rc;
+    RowsInvArgs<double> r{};
+    r.P0 = D.Fx;
+    r.P1 = D.Fy;
+    r.ncols_in = c->ncols_tend;
+    r.w = D.w;
+    r.b = c->adjc_bst[st];
+    r.out_r = c->adjc_g;
+    r.w_out = c->adjc_wbar;
+    r.gscale = 1.0;
+    if ((rc = launch_rows_inv<double, MODE_PLANES, EPI_ADJW>(c, r))) return rc;
+    // F(φ̄) = F(Dy ū - Dx v̄)
+    if ((rc = launch_rows_fwd<double, double, SRC_W>(c, nullptr, c->adjc_wbar, D.Fx, D.Fy, c->ncols_full))) return rc;
+    ColsArgs<double> ph{};
+    ph.inA = D.Fx;
+    ph.inB = D.Fy;
+    ph.outA = c->adjc_phibar;
+    ph.scale = 1.0;
+    if ((rc = launch_cols_args<double, OP_PHIBAR>(c, ph, c->ncols_full, 1))) return rc;
+    // through the linearised Monge-Ampère solve
+    ssg_ma_diag d{};
+    if ((rc = ma_adjoint_solve(c, mp, &d))) return rc;
+    if ((rc = irfft2_device(c, c->adjc_bbar, c->adjc_tmp))) return rc;
+    return lincomb(c, c->adjc_g, 1.0, c->adjc_g, 1.0, c->adjc_tmp);
+}
+
+// Reverse sweep over steps [step0 + 1, step0 + nsteps] with the forward
+// states b(step0 + j) provided by `state_at(j)` (which leaves it in c->b).
+template <typename StateAt>
+int adjoint_coupled_sweep(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* mp, int64_t step0,
+                          int64_t nsteps, StateAt&& state_at) {
+    const double dt = p->dt;
+    int rc;
+    for (int64_t j = nsteps; j >= 1; --j) {
+        const int64_t n = step0 + j;  // step n took b(n-1) to b(n)
+        if (p->filter_freq > 0 && (n % p->filter_freq) == 0)
+            if ((rc = filter_b<double>(c, p->filter_strength, p->filter_order, p->filter_cutoff, 0, c->adj_lam)))
+                return rc;
+        if ((rc = state_at(j - 1))) return rc;
+        if ((rc = coupled_step_record(c, p, mp, n))) return rc;
+        // λ̄ of b(n-1): acc = λ + Σ stage terms
+        if ((rc = lincomb(c, c->adj_acc, 1.0, c->adj_lam, 0.0, nullptr))) return rc;
+        if ((rc = lincomb(c, c->adj_s, 0.75 * dt, c->adj_lam, 0.0, nullptr))) return rc;      // k̄3
+        if ((rc = lincomb(c, c->adjc_k1, 0.25 * dt, c->adj_lam, 0.0, nullptr))) return rc;    // k̄1
+        if ((rc = adjc_stage(c, mp, c->adj_s, 2))) return rc;                                 // b̄2
+        if ((rc = lincomb(c, c->adj_acc, 1.0, c->adj_acc, 1.0, c->adjc_g))) return rc;
+        if ((rc = lincomb(c, c->adj_s, 2.0 * dt / 3.0, c->adjc_g, 0.0, nullptr))) return rc;  // k̄2
+        if ((rc = adjc_stage(c, mp, c->adj_s, 1))) return rc;                                 // b̄1
+        if ((rc = lincomb(c, c->adj_acc, 1.0, c->adj_acc, 1.0, c->adjc_g))) return rc;
+        if ((rc = lincomb(c, c->adjc_k1, 1.0, c->adjc_k1, dt / 3.0, c->adjc_g))) return rc;
+        if ((rc = adjc_stage(c, mp, c->adjc_k1, 0))) return rc;                               // b̄0 part
+        if ((rc = lincomb(c, c->adj_lam, 1.0, c->adj_acc, 1.0, c->adjc_g))) return rc;
+    }
+    return SSG_OK;
+}
+
+// checkpointed adjoint of nsteps coupled steps from the context's current b
+// (at step number step0): checkpoints every `every` steps on the forward
+// pass, then each segment is recomputed from its checkpoint, storing the
+// per-step states, and swept in reverse.
+int adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* mp, int64_t step0, int64_t nsteps,
+                    int every) {
+    const size_t n2 = (size_t)c->nx * c->ny, bytes = n2 * sizeof(double);
+    const int64_t nck = (nsteps + every - 1) / every;
+    double* ck = nullptr;
+    double* seg = nullptr;
+    HIP_OK(c, hipMalloc(&ck, (size_t)nck * bytes));
+    if (hipMalloc(&seg, (size_t)every * bytes) != hipSuccess) {
+        (void)hipFree(ck);
+        return set_err(c, SSG_ERR_HIP, "checkpoint allocation failed");
+    }
+    auto body = [&]() -> int {
+        int rc;
+        // forward pass with checkpoints; c->b is restored at the end
+        for (int64_t j = 0; j < nsteps; ++j) {
+            if (j % every == 0)
+                HIP_OK(c, hipMemcpyAsync(ck + (size_t)(j / every) * n2, c->b, bytes, hipMemcpyDeviceToDevice,
+                                         c->stream));
+            if ((rc = coupled_step_record(c, p, mp, step0 + j + 1))) return rc;
+        }
+        for (int64_t sg = nck - 1; sg >= 0; --sg) {
+            const int64_t j0 = sg * every;
+            const int64_t len = std::min<int64_t>(every, nsteps - j0);
+            // recompute the segment's per-step states from its checkpoint
+            HIP_OK(c, hipMemcpyAsync(c->b, ck + (size_t)sg * n2, bytes, hipMemcpyDeviceToDevice, c->stream));
+            for (int64_t k = 0; k < len; ++k) {
+                HIP_OK(c, hipMemcpyAsync(seg + (size_t)k * n2, c->b, bytes, hipMemcpyDeviceToDevice, c->stream));
+                if (k + 1 < len && (rc = coupled_step_record(c, p, mp, step0 + j0 + k + 1))) return rc;
+            }
+            auto state_at = [&](int64_t jj) -> int {
+                HIP_OK(c, hipMemcpyAsync(c->b, seg + (size_t)jj * n2, bytes, hipMemcpyDeviceToDevice, c->stream));
+                return SSG_OK;
+            };
+            if ((rc = adjoint_coupled_sweep(c, p, mp, step0 + j0, len, state_at))) return rc;
+        }
+        // leave b at the start state
+        HIP_OK(c, hipMemcpyAsync(c->b, ck, bytes, hipMemcpyDeviceToDevice, c->stream));
+        return SSG_OK;
+    };
+    const int rc = body();
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(ck);
+    (void)hipFree(seg);
+    return rc;
+}
+
 int check_domain(const ssg_domain_desc* d, char* msg, size_t len) {
     if (!d) return snprintf(msg, len, "null domain descriptor"), SSG_ERR_ARG;
     if (!pow2_in_range(d->nx) || !pow2_in_range(d->ny))
@@ -1264,6 +1658,7 @@ int ssg_destroy(ssg_ctx* c) {
     (void)hipFree(c->adj_lam);
     (void)hipFree(c->adj_acc);
     (void)hipFree(c->adj_s);
+    adjc_free(c);
     (void)hipFree(c->b);
     (void)hipFree(c->phi);
     (void)hipFree(c->dscratch);
@@ -1515,6 +1910,49 @@ int ssg_adjoint(ssg_ctx* c, const ssg_time_params* p, int64_t step_end, int64_t 
     HIP_OK(c, hipMemcpyAsync(c->adj_lam, lam_in, bytes, hipMemcpyHostToDevice, c->stream));
     if ((rc = adjoint_steps(c, p, step_end, nsteps))) return rc;
     HIP_OK(c, hipMemcpyAsync(lam_out, c->adj_lam, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+int ssg_adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* mp, int64_t step_start,
+                        int64_t nsteps, int32_t checkpoint_every, const double* lam_in, double* lam_out) {
+    if (!c) return SSG_ERR_ARG;
+    if (!p || !mp || !lam_in || !lam_out) return set_err(c, SSG_ERR_ARG, "null pointer");
+    if (p->scheme != SSG_RK3) return set_err(c, SSG_ERR_UNSUPPORTED, "the adjoint covers the RK3 scheme");
+    if (nsteps < 0 || step_start < 0) return set_err(c, SSG_ERR_ARG, "need nsteps >= 0 and step_start >= 0");
+    if (checkpoint_every < 1) return set_err(c, SSG_ERR_ARG, "checkpoint_every must be >= 1");
+    if (!(p->dt > 0)) return set_err(c, SSG_ERR_ARG, "dt must be > 0");
+    int rc = ma_check_params(c, mp);
+    if (rc) return rc;
+    HIP_OK(c, hipSetDevice(c->device));
+    if ((rc = adjc_alloc(c))) return rc;
+    const size_t bytes = (size_t)c->nx * c->ny * sizeof(double);
+    HIP_OK(c, hipMemcpyAsync(c->adj_lam, lam_in, bytes, hipMemcpyHostToDevice, c->stream));
+    if (nsteps > 0 && (rc = adjoint_coupled(c, p, mp, step_start, nsteps, checkpoint_every))) return rc;
+    HIP_OK(c, hipMemcpyAsync(lam_out, c->adj_lam, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+int ssg_ma_adjoint(ssg_ctx* c, const ssg_ma_params* mp, const double* phibar, double* bbar, ssg_ma_diag* diag) {
+    if (!c) return SSG_ERR_ARG;
+    if (!mp || !phibar || !bbar) return set_err(c, SSG_ERR_ARG, "null pointer");
+    int rc = ma_check_params(c, mp);
+    if (rc) return rc;
+    if (!c->ma_ready || !c->ma_have_d)
+        return set_err(c, SSG_ERR_ARG, "no Monge-Ampere solution to linearise about (call ssg_ma_solve first)");
+    HIP_OK(c, hipSetDevice(c->device));
+    if ((rc = adjc_alloc(c))) return rc;
+    const size_t bytes = (size_t)c->nx * c->ny * sizeof(double);
+    HIP_OK(c, hipMemcpyAsync(c->adjc_tmp, phibar, bytes, hipMemcpyHostToDevice, c->stream));
+    if ((rc = rfft2_device(c, c->adjc_tmp, c->adjc_phibar))) return rc;
+    if ((rc = ma_phys_derivs(c))) return rc;
+    ssg_ma_diag d{};
+    rc = ma_adjoint_solve(c, mp, &d);
+    if (diag) *diag = d;
+    if (rc) return rc;
+    if ((rc = irfft2_device(c, c->adjc_bbar, c->adjc_tmp))) return rc;
+    HIP_OK(c, hipMemcpyAsync(bbar, c->adjc_tmp, bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     return SSG_OK;
 }

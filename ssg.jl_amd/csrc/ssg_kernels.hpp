@@ -22,7 +22,7 @@
 
 namespace ssg {
 
-enum { SRC_FLUX = 0, SRC_PAIR = 1 };
+enum { SRC_FLUX = 0, SRC_PAIR = 1, SRC_W = 2 };
 enum { MODE_PAIR = 0, MODE_PLANES = 1 };
 enum {
     EPI_STORE = 0,   // out_r[row] = value
@@ -37,7 +37,10 @@ enum {
     // Dx M s and Dy M s of the stage input s; g = gscale (u Gx + v Gy) = gscale T^T s)
     EPI_ADJ3 = 8,    // acc = lam + g;  s = c2 g
     EPI_ADJ2 = 9,    // acc += g;       s = c3 lam + c1 g
-    EPI_ADJ1 = 10    // lam = acc + g
+    EPI_ADJ1 = 10,   // lam = acc + g
+    // coupled-step adjoint (MODE_PLANES): out_r = u A + v B (the frozen-w
+    // transpose), w_out = (b A, b B) (the adjoint of the velocities)
+    EPI_ADJW = 11
 };
 enum {
     OP_TEND = 0,     // -M (Dx Fx + Dy Fy)
@@ -53,7 +56,9 @@ enum {
     OP_MA_DERIV = 10,  // Monge-Ampere: P -> (Pxx, Pyy, Pxy), P spectral
     OP_MA_DIFF = 11,   // Monge-Ampere: y-FFT of D, residual sums against the previous D
     OP_VEL_SPEC = 12,  // U = -Dy P, V = Dx P with P already spectral in y
-    OP_ADJ = 13        // A = Dx M P, B = Dy M P (transposed tendency, spectral part)
+    OP_ADJ = 13,       // A = Dx M P, B = Dy M P (transposed tendency, spectral part)
+    OP_PHIBAR = 14,    // y-FFT of (U, V) -> Dy U - Dx V, spectral (adjoint of u = -Dy φ, v = Dx φ)
+    OP_MA_ADJ = 15     // y-FFT of three products -> Dxx P1 + Dyy P2 - 2 Dxy P3 on interior levels, residual sums
 };
 
 struct Grid {
@@ -137,11 +142,12 @@ __device__ __forceinline__ R sym_filter(const Grid& g, int c, int s, R strength,
 // x-FFT and stores the two half spectra (kx < ncols) to [kx][y] planes.
 //   SRC_FLUX: line l <-> row y0+l, a = u*b, b = v*b  -> (PA, PB) at row y0+l
 //   SRC_PAIR: line l <-> rows (y0+l, y0+G+l) of f   -> PA at both rows
+//   SRC_W   : line l <-> row y0+l, a = w.x, b = w.y   -> (PA, PB) at row y0+l
 // ---------------------------------------------------------------------------
 template <typename R, typename S>
 struct RowsFwdArgs {
     const S* f;            // SRC_PAIR source, or the buoyancy for SRC_FLUX
-    const cpx<R>* w;       // (u, v) for SRC_FLUX
+    const cpx<R>* w;       // (u, v) for SRC_FLUX and SRC_W
     cpx<R>* PA;
     cpx<R>* PB;
     int ncols;
@@ -209,16 +215,20 @@ __device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs<R, S>& a, const 
     tw.load(t, a.twx);
     C v[E];
     int rowA0, rowB0;
-    if (SRC == SRC_FLUX) {
+    if (SRC == SRC_FLUX || SRC == SRC_W) {
         const int y = blk * G + l;
         rowA0 = rowB0 = blk * G;
         const size_t base = (size_t)nx * y;
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int x = t + T * m;
-            const R bb = (R)a.f[base + x];
             const C uv = a.w[base + x];
-            v[m] = C{uv.x * bb, uv.y * bb};
+            if (SRC == SRC_W) {
+                v[m] = uv;
+            } else {
+                const R bb = (R)a.f[base + x];
+                v[m] = C{uv.x * bb, uv.y * bb};
+            }
         }
     } else {
         const int yA = blk * 2 * G + l, yB = yA + G;
@@ -233,7 +243,7 @@ __device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs<R, S>& a, const 
     C* buf = lds + l * LE;
     F::template run<-1>(v, buf, t, tw);
     F::to_lds(v, buf, t);
-    store_split<R, N, E>(lds, G, a.ncols, a.PA, SRC == SRC_FLUX ? a.PB : a.PA, a.g.pitch, rowA0, rowB0);
+    store_split<R, N, E>(lds, G, a.ncols, a.PA, SRC == SRC_PAIR ? a.PA : a.PB, a.g.pitch, rowA0, rowB0);
 }
 
 template <typename R, typename S, int N, int E, int SRC>
@@ -248,6 +258,7 @@ template <typename R>
 struct ColsArgs {
     const cpx<R>* inA;
     const cpx<R>* inB;
+    const cpx<R>* inC;     // OP_MA_ADJ third input
     cpx<R>* outA;
     cpx<R>* outB;
     int ncols;
@@ -265,6 +276,7 @@ struct ColsArgs {
     double eps;
     int body;               // 1: B2 is the interior right-hand side on every level
     double* partial;        // per-block (sum |eps dD|^2, sum |rhs|^2) over interior levels
+                            // (OP_MA_ADJ: sum |rhs|^2 is that of B2 on the top level)
 };
 
 // per-block sums of W values into out[blockIdx.x * W + k]; blockDim is a
@@ -291,11 +303,12 @@ __device__ __forceinline__ void block_sum_g(double (&v)[W], double* out, int bid
 
 // LDS line buffers per column line for each operation
 __host__ __device__ constexpr int cols_nbuf(int op) {
-    return op == OP_MA_DERIV ? 3
-                             : ((op == OP_TEND || op == OP_TEND_Q || op == OP_JAC || op == OP_VEL || op == OP_VEL_SPEC ||
-                                 op == OP_ADJ)
-                                    ? 2
-                                    : 1);
+    return (op == OP_MA_DERIV || op == OP_MA_ADJ)
+               ? 3
+               : ((op == OP_TEND || op == OP_TEND_Q || op == OP_JAC || op == OP_VEL || op == OP_VEL_SPEC ||
+                   op == OP_ADJ || op == OP_PHIBAR)
+                      ? 2
+                      : 1);
 }
 
 template <typename R, int M, int E, int OP>
@@ -315,7 +328,7 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
     C* const buf1[1] = {lds + (l * NBUF) * LE};
     typename F::Twiddles tw;
     tw.load(t, a.twy);
-    constexpr bool TWO_IN = (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC);
+    constexpr bool TWO_IN = (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC || OP == OP_PHIBAR || OP == OP_MA_ADJ);
     C vv[2][E];
     C (&va)[E] = vv[0];
     C (&vb)[E] = vv[1];
@@ -335,7 +348,64 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
         }
     };
 
-    if constexpr (OP == OP_MA_DERIV) {
+    if constexpr (OP == OP_MA_ADJ) {
+        // transpose of the linearised D operator, D'(Φ)ᵀ λ = Dxx(Φyy λ) +
+        // Dyy(Φxx λ) - 2 Dxy(Φxy λ): the three products arrive x-spectral;
+        // all three symbols are real and even, so each operator is its own
+        // transpose.  Only interior levels carry D.
+        C w3[3][E];
+        const C* srcC = a.inC + (size_t)(live ? line : 0) * pitch;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            w3[0][m] = va[m];
+            w3[1][m] = vb[m];
+            w3[2][m] = live ? srcC[t + T * m] : C{0, 0};
+        }
+        C* const b3[3] = {lds + (l * NBUF) * LE, lds + (l * NBUF + 1) * LE, lds + (l * NBUF + 2) * LE};
+        F::template run_n<-1, 3>(w3, b3, t, tw);
+        const int lev = a.nlev > 1 ? line - c * a.nlev : 0;
+        const bool interior = live && lev >= 1 && lev <= a.nlev - 2;
+        const bool top = live && lev == a.nlev - 1;
+        const double wgt = (c == 0 || 2 * c == a.g.nx) ? 1.0 : 2.0;
+        double acc[2] = {0.0, 0.0};
+        C* dp = a.outA + (size_t)(live ? line : 0) * pitch;
+        const cpx<double>* b2p = a.B2 + (size_t)c * a.g.ny;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int y = t + T * m;
+            const int s = signed_k(y, a.g.ny);
+            C q = C{0, 0};
+            if (interior) {
+                const R sxx = sym_dxx<R>(a.g, c), syy = sym_dyy<R>(a.g, s), sxy = sym_dxy<R>(a.g, c, s);
+                q = C{sxx * w3[0][m].x + syy * w3[1][m].x - (R)2 * sxy * w3[2][m].x,
+                      sxx * w3[0][m].y + syy * w3[1][m].y - (R)2 * sxy * w3[2][m].y};
+                const C prev = dp[y];
+                const double rx = a.eps * ((double)prev.x - (double)q.x);
+                const double ry = a.eps * ((double)prev.y - (double)q.y);
+                acc[0] += wgt * (rx * rx + ry * ry);
+            }
+            if (top) {
+                const cpx<double> bb = b2p[y];
+                acc[1] += wgt * (bb.x * bb.x + bb.y * bb.y);
+            }
+            if (live) dp[y] = q;
+        }
+        block_sum_g<2>(acc, a.partial, bidx);
+        return;
+    } else if constexpr (OP == OP_PHIBAR) {
+        // φ̄ = Dy U - Dx V (u = -Dy φ, v = Dx φ; Dx, Dy are antisymmetric),
+        // left spectral: F(φ̄) = i sdy F(U) - i sdx F(V)
+        C* const b2[2] = {lds + (l * NBUF) * LE, lds + (l * NBUF + 1) * LE};
+        F::template run_n<-1, 2>(vv, b2, t, tw);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int s = signed_k(t + T * m, a.g.ny);
+            const C pu = cmul_i(va[m], sym_dy<R>(a.g, s)), pv = cmul_i(vb[m], sym_dx<R>(a.g, c, s));
+            va[m] = C{pu.x - pv.x, pu.y - pv.y};
+        }
+        store(a.outA, va);
+        return;
+    } else if constexpr (OP == OP_MA_DERIV) {
         C w3[3][E];
 #pragma unroll
         for (int m = 0; m < E; ++m) {
@@ -624,6 +694,11 @@ __device__ __forceinline__ void rows_inv_body(const RowsInvArgs<R>& a, const int
             } else {
                 a.lam[bA + x] = a.acc[bA + x] + g;
             }
+        } else if constexpr (EPI == EPI_ADJW) {
+            const C uv = a.w[bA + x];
+            const double bb = a.b[bA + x];
+            a.out_r[bA + x] = (R)(a.gscale * ((double)uv.x * (double)tA + (double)uv.y * (double)tB));
+            a.w_out[bA + x] = C{(R)(bb * (double)tA), (R)(bb * (double)tB)};
         } else if (EPI == EPI_STORE) {
             a.out_r[bA + x] = tA;
             a.out_r[bB + x] = tB;
@@ -813,6 +888,13 @@ template <typename A, typename B>
 __global__ void k_convert(const A* __restrict__ in, B* __restrict__ out, long n) {
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
         out[q] = (B)in[q];
+}
+
+// out = a x + b y (y may be null)
+// (out may alias x or y: each element is read before it is written)
+__global__ void k_lincomb(double* out, double ca, const double* x, double cb, const double* y, long n) {
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        out[q] = ca * x[q] + (y ? cb * y[q] : 0.0);
 }
 
 // complex array precision change (FP32 <-> FP64)

@@ -20,6 +20,7 @@ struct MaRowsArgs {
     const cpx<R>* Pyy;
     const cpx<R>* Pxy;
     cpx<R>* Dx;
+    R* phys[3];  // MA_ROWS_PHYS: the three fields in physical space, [row][x]
     int ncols;   // nx/2 + 1
     int G;
     int nblk;
@@ -66,7 +67,11 @@ __device__ __forceinline__ void ma_scatter(cpx<R>* lds, int G, int total, const 
     }
 }
 
-template <typename R, int N, int E>
+enum { MA_ROWS_D = 0, MA_ROWS_PHYS = 1 };
+
+// MA_ROWS_D: D = Φxx Φyy - Φxy² and its x-FFT; MA_ROWS_PHYS: the three
+// derivative fields stored in physical space (for the adjoint)
+template <typename R, int N, int E, int MODE = MA_ROWS_D>
 __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
     using C = cpx<R>;
     using F = LineFFT<R, N, E>;
@@ -95,6 +100,17 @@ __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
     ma_scatter<R, N, E, MAXIT>(lds + 2 * LE, G, total, qc0, qc1, 3);
     C w3[3][E];
     F::template run_n_from_lds<+1, 3>(w3, b3, t, tw);
+    if constexpr (MODE == MA_ROWS_PHYS) {
+        const size_t rA = (size_t)(y0 + l) * a.g.nx, rB = (size_t)(y0 + G + l) * a.g.nx;
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                a.phys[f][rA + t + T * m] = w3[f][m].x;
+                a.phys[f][rB + t + T * m] = w3[f][m].y;
+            }
+        return;
+    }
     // D = Φxx Φyy - Φxy² for rows A (re) and B (im): compute_d_operator!
     // (src/poisson.jl:183-223)
     C v[E];
@@ -111,6 +127,75 @@ __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
     __syncthreads();
     // line l's result sits in buffer 3l: store_split with a stride-3 view
     store_split_stride<R, N, E>(lds, G, 3, a.ncols, a.Dx, a.Dx, pitch, y0, y0 + G);
+}
+
+// Adjoint row pass: λ (x-spectral) -> physical; the products Φyy λ, Φxx λ,
+// Φxy λ with the stored physical derivative fields of the linearisation
+// point; x-FFT of the three products.
+template <typename R>
+struct MaAdjRowsArgs {
+    const cpx<R>* lam;      // [kx][row], pitch = nz * ny
+    const R* pxx;           // physical [row][x]
+    const R* pyy;
+    const R* pxy;
+    cpx<R>* out[3];         // x-spectra of Φyy λ, Φxx λ, Φxy λ
+    int ncols;
+    int G;
+    int nblk;
+    const cpx<R>* twx;
+    Grid g;
+};
+
+template <typename R, int N, int E>
+__global__ __launch_bounds__(256) void k_ma_adj_rows(MaAdjRowsArgs<R> a) {
+    using C = cpx<R>;
+    using F = LineFFT<R, N, E>;
+    constexpr int T = F::T;
+    constexpr int LE = lds_line_elems(N);
+    constexpr int MAXIT = (N / 2 + 1 + T - 1) / T;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int l = threadIdx.x / T, t = threadIdx.x - l * T;
+    const int G = a.G;
+    const int blk = xcd_block(blockIdx.x, a.nblk);
+    const int y0 = blk * 2 * G;
+    const int pitch = a.g.pitch;
+    const int total = a.ncols * G;
+    typename F::Twiddles tw;
+    tw.load(t, a.twx);
+    const size_t rA = (size_t)(y0 + l) * a.g.nx, rB = (size_t)(y0 + G + l) * a.g.nx;
+    R fA[3][E], fB[3][E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int x = t + T * m;
+        fA[0][m] = a.pyy[rA + x];
+        fB[0][m] = a.pyy[rB + x];
+        fA[1][m] = a.pxx[rA + x];
+        fB[1][m] = a.pxx[rB + x];
+        fA[2][m] = a.pxy[rA + x];
+        fB[2][m] = a.pxy[rB + x];
+    }
+    C q0[MAXIT], q1[MAXIT];
+    ma_load<R, N, E, MAXIT>(a.lam, pitch, y0, G, total, q0, q1);
+    ma_scatter<R, N, E, MAXIT>(lds, G, total, q0, q1, 3);
+    C* const b3[3] = {lds + (3 * l) * LE, lds + (3 * l + 1) * LE, lds + (3 * l + 2) * LE};
+    C v[E];
+    F::template run_from_lds<+1>(v, b3[0], t, tw);
+    C w3[3][E];
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+        for (int m = 0; m < E; ++m) w3[f][m] = C{fA[f][m] * v[m].x, fB[f][m] * v[m].y};
+    F::template run_n<-1, 3>(w3, b3, t, tw);
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+        for (int m = 0; m < E; ++m) b3[f][lpad(t + T * m)] = w3[f][m];
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+        store_split_stride<R, N, E>(lds + f * LE, G, 3, a.ncols, a.out[f], a.out[f], pitch, y0, y0 + G);
 }
 
 // Per-level stencil data, 0-based levels, nz >= 3 (src/transforms.jl:798-850,
@@ -206,6 +291,128 @@ __global__ __launch_bounds__(128) void k_ma_tridiag(MaTriArgs<R, P> a) {
             a.Phi[base + (size_t)k * ny] = cpx<P>{(P)x.x, (P)x.y};
         }
     }
+}
+
+}  // namespace ssg
+
+namespace ssg {
+
+// Transpose of the per-wavenumber solve Φ̂ = L⁻¹ f̃ of k_ma_tridiag.  That
+// solve is A' Φ = E f̃ with A' tridiagonal (the one-sided Neumann rows after
+// the elimination through their adjacent interior rows) and E the
+// elimination (r0 = -C0 f1/γ1, r_top = g - ct f_{n-2}/α_{n-2}; the gauged
+// mean mode replaces row 0 by Φ0 = 0), so its transpose is Eᵀ A'⁻ᵀ.
+// Input y per level: ε q on interior levels (use_q) plus φ̄ on the top
+// level; output F̄ = Eᵀ A'⁻ᵀ y per row (interior rows carry f, the top row g)
+// and b̄ = Σ interior F̄ (body variant) or F̄_top (surface variant).
+struct MaTriAdjArgs {
+    const cpx<double>* q;       // [kx][lev][ky]
+    const cpx<double>* phibar;  // [kx][ky]
+    cpx<double>* Fbar;          // [kx][lev][ky]
+    cpx<double>* bbar;          // [kx][ky]
+    const double* coef;
+    int nz;
+    int ncols;
+    double eps;
+    int body;
+    int top;
+    int use_q;
+    Grid g;
+};
+
+template <int NZMAX>
+__global__ __launch_bounds__(128) void k_ma_tridiag_adj(MaTriAdjArgs a) {
+    const long total = (long)a.ncols * a.g.ny;
+    const long qi = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (qi >= total) return;
+    const int ny = a.g.ny, nz = a.nz;
+    const int kx = (int)(qi / ny), ky = (int)(qi - (long)kx * ny);
+    const int s = signed_k(ky, ny);
+    const double hs = (double)sym_dxx<double>(a.g, kx) + (double)sym_dyy<double>(a.g, s);
+    const double* cf = a.coef;
+    const size_t base = (size_t)kx * nz * ny + ky;
+    const bool gauge = (kx == 0 && ky == 0);
+    // rows of A': D_k diagonal, L_k sub-diagonal (coefficient of Φ_{k-1}),
+    // U_k super-diagonal (coefficient of Φ_{k+1})
+    const double A0 = cf[3 * nz], B0 = cf[3 * nz + 1], C0 = cf[3 * nz + 2];
+    const double ga1 = cf[5];
+    const double D0 = gauge ? 1.0 : A0 - C0 * cf[3] / ga1;
+    const double U0 = gauge ? 0.0 : B0 - C0 * (cf[4] + hs) / ga1;
+    const int ki = nz - 2;
+    const double alT = cf[3 * ki], beT = cf[3 * ki + 1] + hs, gaT = cf[3 * ki + 2];
+    const double at = cf[3 * nz + 3], bt = cf[3 * nz + 4], ct = cf[3 * nz + 5];
+    const double LN = bt - ct * beT / alT, DN = at - ct * gaT / alT;
+    auto Dk = [&](int k) -> double { return k == 0 ? D0 : (k == nz - 1 ? DN : cf[3 * k + 1] + hs); };
+    auto Lk = [&](int k) -> double { return k == nz - 1 ? LN : cf[3 * k]; };       // 1 <= k <= nz-1
+    auto Uk = [&](int k) -> double { return k == 0 ? U0 : cf[3 * k + 2]; };        // 0 <= k <= nz-2
+    auto yk = [&](int k) -> double2 {
+        double2 y = {0.0, 0.0};
+        if (a.use_q && k >= 1 && k <= nz - 2) {
+            const cpx<double> qq = a.q[base + (size_t)k * ny];
+            y.x = a.eps * qq.x;
+            y.y = a.eps * qq.y;
+        }
+        if (k == nz - 1) {
+            const cpx<double> pb = a.phibar[(size_t)kx * ny + ky];
+            y.x += pb.x;
+            y.y += pb.y;
+        }
+        return y;
+    };
+    // Thomas on A'ᵀ: row k has sub U_{k-1}, diagonal D_k, super L_{k+1}
+    double cp[NZMAX];
+    double2 dp[NZMAX];
+    {
+        const double d0 = Dk(0);
+        const double2 y0 = yk(0);
+        cp[0] = Lk(1) / d0;
+        dp[0] = {y0.x / d0, y0.y / d0};
+    }
+#pragma unroll
+    for (int k = 1; k < NZMAX; ++k) {
+        if (k < nz) {
+            const double sub = Uk(k - 1);
+            const double m = Dk(k) - sub * cp[k - 1];
+            const double2 y = yk(k);
+            cp[k] = (k + 1 < nz) ? Lk(k + 1) / m : 0.0;
+            dp[k] = {(y.x - sub * dp[k - 1].x) / m, (y.y - sub * dp[k - 1].y) / m};
+        }
+    }
+    // back substitution (μ overwrites dp); F̄ = Eᵀ μ adds the eliminations'
+    // transposes: F̄_1 -= (C0/γ1) μ_0, F̄_{n-2} -= (ct/α_{n-2}) μ_{n-1}
+    double2 corr = {0.0, 0.0};
+    double2 x = {0.0, 0.0};
+#pragma unroll
+    for (int k = NZMAX - 1; k >= 0; --k) {
+        if (k <= nz - 1) {
+            x = {dp[k].x - cp[k] * x.x, dp[k].y - cp[k] * x.y};
+            dp[k] = x;
+            if (k == nz - 1) corr = {ct / alT * x.x, ct / alT * x.y};
+        }
+    }
+    double2 (&mu)[NZMAX] = dp;
+    if (!gauge) {
+        mu[1].x -= C0 / ga1 * mu[0].x;
+        mu[1].y -= C0 / ga1 * mu[0].y;
+    }
+    double2 bsum = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NZMAX; ++k) {
+        if (k <= nz - 1) {
+            double2 f = mu[k];
+            if (k == nz - 2) {
+                f.x -= corr.x;
+                f.y -= corr.y;
+            }
+            a.Fbar[base + (size_t)k * ny] = f;
+            if (a.body && k >= 1 && k <= nz - 2) {
+                bsum.x += f.x;
+                bsum.y += f.y;
+            }
+            if (a.top && k == nz - 1) bsum = f;
+        }
+    }
+    a.bbar[(size_t)kx * ny + ky] = bsum;
 }
 
 }  // namespace ssg

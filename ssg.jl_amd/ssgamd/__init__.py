@@ -52,6 +52,7 @@ EXPORTED_SYMBOLS = (
     "ssg_profile", "ssg_profile_read", "ssg_tuning", "ssg_ma_solve", "ssg_get_phi3d", "ssg_set_coupling",
     "ssg_last_ma_diag", "ssg_copy_probe", "ssg_adjoint", "ssg_ensemble_create", "ssg_ensemble_destroy",
     "ssg_ensemble_member", "ssg_ensemble_last_error", "ssg_ensemble_timestep", "ssg_ensemble_tuning",
+    "ssg_ma_adjoint", "ssg_adjoint_coupled",
 )
 
 MA_BODY = 0      # as wired: b on every level, surface slope 0
@@ -161,6 +162,9 @@ def load_library(path: str | None = None):
                                           ctypes.c_int64]
     lib.ssg_ensemble_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), _dp]
     lib.ssg_profile_read.argtypes = [vp, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), _dp]
+    lib.ssg_ma_adjoint.argtypes = [vp, ctypes.POINTER(_MaParams), _dp, _dp, ctypes.POINTER(_MaDiag)]
+    lib.ssg_adjoint_coupled.argtypes = [vp, ctypes.POINTER(_TimeParams), ctypes.POINTER(_MaParams), ctypes.c_int64,
+                                        ctypes.c_int64, ctypes.c_int32, _dp, _dp]
     for n in EXPORTED_SYMBOLS:
         getattr(lib, n).restype = ctypes.c_char_p if n in ("ssg_last_error", "ssg_ensemble_last_error") else ctypes.c_int
     lib.ssg_ensemble_member.restype = vp
@@ -452,6 +456,36 @@ def set_coupling(domain: Domain, *, eps: float | None = 0.1, tol: float = 1e-10,
         return
     p = _ma_params(eps, tol, maxiter, variant, warm)
     domain.check(domain.lib.ssg_set_coupling(domain.handle, ctypes.byref(p)))
+
+
+def ma_adjoint(domain: Domain, phibar, *, eps: float = 0.1, tol: float = 1e-10, maxiter: int = 50,
+               variant: int = MA_BODY) -> tuple:
+    """Vector-Jacobian product of the last Monge-Ampère solve:
+    (∂φₛ/∂bₛ)ᵀ phibar, and the transposed iteration's diagnostics."""
+    phibar = _f64(phibar, (domain.Ny, domain.Nx))
+    out = np.empty_like(phibar)
+    p = _ma_params(eps, tol, maxiter, variant, False)
+    d = _MaDiag()
+    domain.check(domain.lib.ssg_ma_adjoint(domain.handle, ctypes.byref(p), phibar.ctypes.data_as(_dp),
+                                           out.ctypes.data_as(_dp), ctypes.byref(d)))
+    return out, d.as_dict()
+
+
+def timestep_adjoint_coupled(domain: Domain, params: TimeParams, step_start: int, nsteps: int, lam, *,
+                             eps: float = 0.1, tol: float = 1e-10, maxiter: int = 50, variant: int = MA_BODY,
+                             warm: bool = True, checkpoint_every: int = 10) -> np.ndarray:
+    """Vector-Jacobian product of ``nsteps`` coupled RK3 steps (Monge-Ampère
+    solve per stage) from the domain's current ``bs`` at step ``step_start``:
+    returns (∂b_end/∂b_start)ᵀ lam.  Checkpoints every ``checkpoint_every``
+    steps; ``bs`` is left unchanged."""
+    lam = _f64(lam, (domain.Ny, domain.Nx))
+    out = np.empty_like(lam)
+    p = params._c()
+    mp = _ma_params(eps, tol, maxiter, variant, warm)
+    domain.check(domain.lib.ssg_adjoint_coupled(domain.handle, ctypes.byref(p), ctypes.byref(mp), int(step_start),
+                                                int(nsteps), int(checkpoint_every), lam.ctypes.data_as(_dp),
+                                                out.ctypes.data_as(_dp)))
+    return out
 
 
 def timestep_adjoint(domain: Domain, params: TimeParams, step_end: int, nsteps: int, lam) -> np.ndarray:

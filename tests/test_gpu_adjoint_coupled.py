@@ -1,0 +1,133 @@
+"""Discrete adjoint of the coupled step (Monge-Ampère solve at every RK
+stage) and of the Monge-Ampère solve itself, through the C ABI.
+
+Pins: (1) at ε = 0 the solve is linear, so the transposed solve must satisfy
+the dot-product identity ⟨φ̄, φₛ(δb)⟩ = ⟨(∂φₛ/∂b)ᵀ φ̄, δb⟩ to roundoff, for
+both the body-forced and the surface-forced variant (this pins the
+transposed tridiagonal solve, the boundary-row eliminations and the gauge
+of the mean mode); (2) at ε > 0 the vector-Jacobian product of the solve
+against central differences of converged GPU solves; (3) the gradient of
+J = ½∫(b(T) - b_tgt)² dA through coupled RK3 steps, with checkpointing,
+against central differences of GPU forward runs."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def smooth(n, seed, k0=4.0, amp=1.0):
+    from ssgamd.ics import spectral_b0
+    return amp * spectral_b0(n, n, seed=seed, k0=k0)
+
+
+def solve_top(gpu, d, f, b, eps, variant, tol=1e-13):
+    f.bs = b
+    diag = gpu.solve_monge_ampere(f, d, eps=eps, tol=tol, maxiter=200, variant=variant)
+    assert diag["converged"], diag
+    return f.phis
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_ma_adjoint_dot_product_linear(gpu, variant):
+    n, nz = 32, 6
+    d = gpu.make_domain(n, n, nz)
+    f = gpu.allocate_fields(d)
+    rng = np.random.default_rng(3)
+    db = rng.standard_normal((n, n))
+    pbar = rng.standard_normal((n, n))
+    phi = solve_top(gpu, d, f, db, 0.0, variant)
+    bbar, diag = gpu.ma_adjoint(d, pbar, eps=0.0, tol=1e-13, variant=variant)
+    lhs, rhs = np.sum(pbar * phi), np.sum(bbar * db)
+    print("linear dot product", variant, lhs, rhs, flush=True)
+    assert abs(lhs - rhs) < 1e-12 * np.sqrt(np.sum(pbar ** 2) * np.sum(phi ** 2)) + 1e-12 * abs(lhs)
+    d.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_ma_adjoint_against_fd(gpu, variant):
+    n, nz, eps, h = 64, 8, 0.1, 1e-5
+    b = smooth(n, 1, amp=1.0 if variant == 0 else 0.2)
+    d = gpu.make_domain(n, n, nz)
+    f = gpu.allocate_fields(d)
+    rng = np.random.default_rng(4)
+    pbar = smooth(n, 7)
+    solve_top(gpu, d, f, b, eps, variant)
+    bbar, diag = gpu.ma_adjoint(d, pbar, eps=eps, tol=1e-13, maxiter=200, variant=variant)
+    print("adjoint MA iterations", diag["iterations"], diag["relative_residual"], flush=True)
+    for k in range(3):
+        v = rng.standard_normal((n, n))
+        jp = np.sum(pbar * solve_top(gpu, d, f, b + h * v, eps, variant))
+        jm = np.sum(pbar * solve_top(gpu, d, f, b - h * v, eps, variant))
+        fd = (jp - jm) / (2 * h)
+        an = np.sum(bbar * v)
+        print(f"variant {variant} direction {k}: adjoint {an:.12e} fd {fd:.12e} rel {abs(fd - an) / abs(an):.2e}",
+              flush=True)
+        assert abs(fd - an) <= 1e-6 * abs(an)
+    d.close()
+
+
+def coupled_forward(gpu, d, f, b0, params, nsteps, eps, variant):
+    f.bs = b0
+    gpu.set_coupling(d, eps=eps, tol=1e-13, maxiter=200, variant=variant, warm=True)
+    st = gpu.TimeState(0.0, f)
+    gpu.timestep(f, d, params, st, nsteps)
+    gpu.set_coupling(d, eps=None)
+    return f.bs
+
+
+def coupled_gradient_check(gpu, n, nz, nsteps, dt, every, ndir, filter_freq, seed_dirs):
+    eps = 0.1
+    b0 = smooth(n, 0, k0=6.0)
+    b_tgt = smooth(n, 1, k0=6.0)
+    dA = (2 * np.pi / n) ** 2
+    params = gpu.TimeParams(dt, scheme=gpu.RK3, filter_freq=filter_freq, filter_strength=36.0, filter_order=8,
+                            filter_cutoff=0.65)
+    d = gpu.make_domain(n, n, nz)
+    f = gpu.allocate_fields(d)
+    bT = coupled_forward(gpu, d, f, b0, params, nsteps, eps, gpu.MA_BODY)
+    f.bs = b0
+    grad = gpu.timestep_adjoint_coupled(d, params, 0, nsteps, (bT - b_tgt) * dA, eps=eps, tol=1e-13, maxiter=200,
+                                        variant=gpu.MA_BODY, warm=True, checkpoint_every=every)
+    assert np.abs(f.bs - b0).max() == 0.0   # the context is left at the start state
+    rng = np.random.default_rng(seed_dirs)
+    h = 1e-6
+    worst = 0.0
+    for k in range(ndir):
+        v = rng.standard_normal((n, n))
+        v /= np.sqrt(np.mean(v * v))
+        rp = coupled_forward(gpu, d, f, b0 + h * v, params, nsteps, eps, gpu.MA_BODY) - b_tgt
+        rm = coupled_forward(gpu, d, f, b0 - h * v, params, nsteps, eps, gpu.MA_BODY) - b_tgt
+        fd = 0.5 * np.sum((rp - rm) * (rp + rm)) * dA / (2 * h)
+        an = np.sum(grad * v)
+        rel = abs(fd - an) / abs(an)
+        worst = max(worst, rel)
+        print(f"{n}^2x{nz} {nsteps} steps, direction {k}: adjoint {an:.12e} fd {fd:.12e} rel {rel:.2e}", flush=True)
+    d.close()
+    return worst
+
+
+def test_coupled_adjoint_small_with_segments(gpu):
+    """64²×6, 7 steps, checkpoints every 3 (segments of 3, 3, 1), filter every 2."""
+    worst = coupled_gradient_check(gpu, 64, 6, 7, 2e-3, 3, 3, 2, 5)
+    assert worst <= 1e-6
+
+
+def test_coupled_adjoint_configuration(gpu):
+    """The adjoint configuration with the Monge-Ampère solve in the step:
+    256²×16, ε = 0.1, RK3 dt = 1e-3, 100 steps, checkpoint every 10 steps
+    with recompute; central differences along 8 random unit-RMS directions,
+    step 1e-6, FP64."""
+    worst = coupled_gradient_check(gpu, 256, 16, 100, 1e-3, 10, 8, 10, 2)
+    assert worst <= 1e-6
+
+
+def test_coupled_adjoint_rejects(gpu):
+    d = gpu.make_domain(32, 32, 4)
+    with pytest.raises(gpu.SSGError):
+        gpu.timestep_adjoint_coupled(d, gpu.TimeParams(1e-3, scheme=gpu.AB2_LowStorage), 0, 2, np.zeros((32, 32)))
+    with pytest.raises(gpu.SSGError):
+        gpu.timestep_adjoint_coupled(d, gpu.TimeParams(1e-3, scheme=gpu.RK3), 0, 2, np.zeros((32, 32)),
+                                     checkpoint_every=0)
+    with pytest.raises(gpu.SSGError):   # no solution to linearise about yet
+        gpu.ma_adjoint(d, np.zeros((32, 32)))
+    d.close()
