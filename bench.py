@@ -160,6 +160,100 @@ def measure_ab2(ssg, b0, psi, nx, ny, nz, dt, steps):
     return out
 
 
+def measure_adjoint(ssg, steps=100, every=10):
+    """Adjoint configuration, 256²×16, ε = 0.1, RK3 dt = 1e-3, J = ½∫(b(T) -
+    b_tgt)² dA: wall time of one gradient (vector-Jacobian product through
+    `steps` steps) for the frozen-φₛ step and for the coupled step
+    (Monge-Ampère solve per stage, checkpoints every `every` steps), beside
+    the forward run's time."""
+    from ssgamd.ics import spectral_b0, streamfunction_from_b
+    n, nz, dt, eps = 256, 16, 1e-3, 0.1
+    b0 = spectral_b0(n, n, seed=0)
+    b_tgt = spectral_b0(n, n, seed=1)
+    dA = (2 * np.pi / n) ** 2
+    params = ssg.TimeParams(dt, scheme=ssg.RK3, filter_freq=10, filter_strength=36.0, filter_order=8,
+                            filter_cutoff=0.65)
+    out = {"grid": f"{n}x{n}x{nz}", "steps": steps, "checkpoint_every": every, "eps": eps}
+    d = ssg.make_domain(n, n, nz)
+    f = ssg.allocate_fields(d)
+    # frozen φₛ
+    f.phis = streamfunction_from_b(b0)
+    f.bs = b0
+    st = ssg.TimeState(0.0, f)
+    t0 = time.perf_counter()
+    ssg.timestep(f, d, params, st, steps)
+    bT = f.bs
+    t1 = time.perf_counter()
+    ssg.timestep_adjoint(d, params, steps, steps, (bT - b_tgt) * dA)
+    t2 = time.perf_counter()
+    out["frozen_forward_s"] = t1 - t0
+    out["frozen_adjoint_s"] = t2 - t1
+    # coupled
+    f.bs = b0
+    ssg.solve_monge_ampere(f, d, eps=eps, tol=1e-10, maxiter=100, variant=ssg.MA_BODY)
+    ssg.set_coupling(d, eps=eps, tol=1e-10, maxiter=100, variant=ssg.MA_BODY, warm=True)
+    st = ssg.TimeState(0.0, f)
+    t0 = time.perf_counter()
+    ssg.timestep(f, d, params, st, steps)
+    bT = f.bs
+    t1 = time.perf_counter()
+    ssg.set_coupling(d, eps=None)
+    f.bs = b0
+    g = ssg.timestep_adjoint_coupled(d, params, 0, steps, (bT - b_tgt) * dA, eps=eps, tol=1e-10, maxiter=100,
+                                     variant=ssg.MA_BODY, warm=True, checkpoint_every=every)
+    t2 = time.perf_counter()
+    d.close()
+    out["coupled_forward_s"] = t1 - t0
+    out["coupled_adjoint_s"] = t2 - t1
+    out["coupled_adjoint_over_forward"] = (t2 - t1) / (t1 - t0)
+    out["grad_norm"] = float(np.linalg.norm(g))
+    return out
+
+
+def measure_stiff(ssg, steps, adj_steps):
+    """Large stiff configuration: 2048²×64, ε = 0.3, spectrum peak k0 = 12,
+    seed 7, RK3 dt = 1e-4: φₛ from the body-forced Monge-Ampère solve at
+    ε = 0.3, FP32-mixed forward for `steps` steps, then the FP64 adjoint of
+    the last `adj_steps` steps."""
+    from ssgamd.ics import spectral_b0
+    n, nz, dt, eps = 2048, 64, 1e-4, 0.3
+    b0 = spectral_b0(n, n, seed=7, k0=12.0)
+    params = ssg.TimeParams(dt, scheme=ssg.RK3, filter_freq=10, filter_strength=36.0, filter_order=8,
+                            filter_cutoff=0.65)
+    out = {"grid": f"{n}x{n}x{nz}", "eps": eps, "k0": 12, "steps": steps, "adjoint_steps": adj_steps}
+    d = ssg.make_domain(n, n, nz, precision=ssg.SSG_FP32_MIXED)
+    f = ssg.allocate_fields(d)
+    f.bs = b0
+    diag = ssg.solve_monge_ampere(f, d, eps=eps, tol=1e-10, maxiter=100, variant=ssg.MA_BODY)
+    out["ma"] = {"converged": diag["converged"], "iterations": diag["iterations"],
+                 "iterations_fp32": diag["iterations_fp32"], "relative_residual": diag["relative_residual"],
+                 "ms": diag["solve_time"] * 1e3}
+    if not diag["converged"]:
+        # rms-1 forcing with k0 = 12 at ε = 0.3 is outside the elliptic range
+        # (ε·eig(∇∇Φ) > 1 along the iterates): no solution to freeze, so φₛ
+        # falls back to the ε = 0 inversion
+        from ssgamd.ics import streamfunction_from_b
+        f.phis = streamfunction_from_b(b0)
+        out["phis"] = "eps=0 inversion (the eps=0.3 problem is not elliptic)"
+    else:
+        out["phis"] = "Monge-Ampere solution at eps=0.3"
+    st = ssg.TimeState(0.0, f)
+    ssg.timestep(f, d, params, st, 10)
+    d.sync()
+    t0 = time.perf_counter()
+    ssg.timestep(f, d, params, st, steps - 10)
+    d.sync()
+    out["fp32_steps_per_s"] = (steps - 10) / (time.perf_counter() - t0)
+    lam = f.bs * (2 * np.pi / n) ** 2
+    t0 = time.perf_counter()
+    g = ssg.timestep_adjoint(d, params, steps, adj_steps, lam)
+    out["adjoint_s"] = time.perf_counter() - t0
+    out["adjoint_steps_per_s"] = adj_steps / out["adjoint_s"]
+    out["grad_finite"] = bool(np.isfinite(g).all())
+    d.close()
+    return out
+
+
 def measure_ensemble(ssg, steps, warmup):
     """Batched configuration: 64 members (32 x 128², 24 x 256², 8 x 512²),
     ICs from seeds 0-63, per-member ε ~ U[0.05, 0.2] (SplitMix64, stream 1)
@@ -243,6 +337,10 @@ def main():
                     help="also time this many coupled steps (Monge-Ampère solve per stage); 0 = skip")
     ap.add_argument("--ab2-steps", type=int, default=1000,
                     help="also time this many AB2 steps per precision; 0 = skip")
+    ap.add_argument("--adjoint-steps", type=int, default=100,
+                    help="also time one gradient through this many steps of the adjoint configuration; 0 = skip")
+    ap.add_argument("--stiff-steps", type=int, default=2000,
+                    help="also run the 2048²×64 stiff configuration for this many FP32-mixed steps; 0 = skip")
     ap.add_argument("--ensemble-steps", type=int, default=100,
                     help="also time the 64-member batched configuration for this many steps; 0 = skip")
     ap.add_argument("--fft-compare", action="store_true",
@@ -337,6 +435,12 @@ def main():
                                       "phis_rel_l2": rel_l2(f32["phis"], f64["phis"]),
                                       "ke_surface_rel": abs(f32["ke"] - f64["ke"]) / abs(f64["ke"]),
                                       "steps": args.coupled_steps + 1}
+    if args.adjoint_steps > 0 and not args.only:
+        log(f"bench: adjoint configuration 256x256x16, {args.adjoint_steps} steps")
+        out["adjoint"] = measure_adjoint(ssg, args.adjoint_steps)
+    if args.stiff_steps > 0 and not args.only:
+        log(f"bench: stiff configuration 2048x2048x64, {args.stiff_steps} steps + adjoint of the last 50")
+        out["stiff"] = measure_stiff(ssg, args.stiff_steps, 50)
     if args.ensemble_steps > 0 and not args.only:
         log(f"bench: ensemble 64 members, {args.ensemble_steps} steps")
         out["ensemble_fp32"] = measure_ensemble(ssg, args.ensemble_steps, 20)
