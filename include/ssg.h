@@ -22,6 +22,7 @@
 #ifndef SSG_H
 #define SSG_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -198,6 +199,24 @@ int ssg_ma_adjoint(ssg_ctx* ctx, const ssg_ma_params* params, const double* phib
    re-solved per stage. */
 int ssg_adjoint_coupled(ssg_ctx* ctx, const ssg_time_params* params, const ssg_ma_params* ma, int64_t step_start,
                         int64_t nsteps, int32_t checkpoint_every, const double* lam_in, double* lam_out);
+
+/* set_b! (src/setICs.jl:21-88): b with its k = 0 mode removed and the 2/3
+   mask applied (rfft_2d!, dealias_2d!, irfft_2d!), then, when params is not
+   NULL, solve_monge_ampere_fields! for φₛ and the surface velocities
+   (compute_surface_geostrophic_velocities!).  SSG_ERR_NOT_CONVERGED if the
+   solve misses params->tol (b and φₛ are still set). */
+int ssg_set_b(ssg_ctx* ctx, const double* host, int64_t n, const ssg_ma_params* params, ssg_ma_diag* diag);
+/* compute_cfl_number (src/timestep.jl:91-117) on the surface velocities:
+   cfl = max(max|u| dt/dx, max|v| dt/dy); umax, vmax optional. */
+int ssg_cfl(ssg_ctx* ctx, double dt, double* cfl, double* umax, double* vmax);
+/* Exact restart (no counterpart in the reference, whose checkpoint keeps b
+   and φ but not the AB2 history, src/output.jl:356-445): the step state
+   (b, φₛ, the AB2 history, t, step) as a byte blob.  Loading it into a
+   context of the same grid and precision resumes the frozen-φₛ step
+   bitwise. */
+int ssg_state_size(ssg_ctx* ctx, size_t* bytes);
+int ssg_save_state(ssg_ctx* ctx, const ssg_time_state* state, void* buf, size_t bytes);
+int ssg_load_state(ssg_ctx* ctx, const void* buf, size_t bytes, ssg_time_state* state);
 
 /* diagnostics of the most recent Monge-Ampère solve */
 int ssg_last_ma_diag(ssg_ctx* ctx, ssg_ma_diag* diag);

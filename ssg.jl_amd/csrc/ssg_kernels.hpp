@@ -58,7 +58,8 @@ enum {
     OP_VEL_SPEC = 12,  // U = -Dy P, V = Dx P with P already spectral in y
     OP_ADJ = 13,       // A = Dx M P, B = Dy M P (transposed tendency, spectral part)
     OP_PHIBAR = 14,    // y-FFT of (U, V) -> Dy U - Dx V, spectral (adjoint of u = -Dy φ, v = Dx φ)
-    OP_MA_ADJ = 15     // y-FFT of three products -> Dxx P1 + Dyy P2 - 2 Dxy P3 on interior levels, residual sums
+    OP_MA_ADJ = 15,    // y-FFT of three products -> Dxx P1 + Dyy P2 - 2 Dxy P3 on interior levels, residual sums
+    OP_SETB = 16       // M P with the k = 0 mode removed (set_b!'s zero mean and dealias)
 };
 
 struct Grid {
@@ -525,6 +526,9 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
                     va[m] = cmul_i(va[m], sym_dy<R>(a.g, s) * a.scale);
                 } else if constexpr (OP == OP_MASK) {
                     va[m] = cscale(va[m], sym_mask<R>(a.g, c, s) * a.scale);
+                } else if constexpr (OP == OP_SETB) {
+                    const R mk = (c == 0 && s == 0) ? (R)0 : sym_mask<R>(a.g, c, s);
+                    va[m] = cscale(va[m], mk * a.scale);
                 }
             }
             F::template run<+1>(va, buf1[0], t, tw);
@@ -850,7 +854,7 @@ __device__ __forceinline__ void block_sum(double (&v)[W], double* out) {
 }
 
 // sum b, sum b^2, sum non-finite count
-__global__ __launch_bounds__(256) void k_stats(const double* __restrict__ b, long n, double* partial) {
+static __global__ __launch_bounds__(256) void k_stats(const double* __restrict__ b, long n, double* partial) {
     double v[3] = {0, 0, 0};
     for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
         const double x = b[q];
@@ -890,9 +894,34 @@ __global__ void k_convert(const A* __restrict__ in, B* __restrict__ out, long n)
         out[q] = (B)in[q];
 }
 
+// per-block max |u| and max |v| of the interleaved velocities
+static __global__ __launch_bounds__(256) void k_wmax(const double2* __restrict__ w, long n, double* partial) {
+    __shared__ double red[2][256];
+    double mu = 0.0, mv = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const double2 z = w[q];
+        mu = fmax(mu, fabs(z.x));
+        mv = fmax(mv, fabs(z.y));
+    }
+    red[0][threadIdx.x] = mu;
+    red[1][threadIdx.x] = mv;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + st]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = red[0][0];
+        partial[2 * blockIdx.x + 1] = red[1][0];
+    }
+}
+
 // out = a x + b y (y may be null)
 // (out may alias x or y: each element is read before it is written)
-__global__ void k_lincomb(double* out, double ca, const double* x, double cb, const double* y, long n) {
+static __global__ void k_lincomb(double* out, double ca, const double* x, double cb, const double* y, long n) {
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
         out[q] = ca * x[q] + (y ? cb * y[q] : 0.0);
 }

@@ -70,7 +70,11 @@ __device__ __forceinline__ void ma_scatter(cpx<R>* lds, int G, int total, const 
 enum { MA_ROWS_D = 0, MA_ROWS_PHYS = 1 };
 
 // MA_ROWS_D: D = Φxx Φyy - Φxy² and its x-FFT; MA_ROWS_PHYS: the three
-// derivative fields stored in physical space (for the adjoint)
+// derivative fields stored in physical space (for the adjoint).  The three
+// inverse transforms run one after another through a single LDS line
+// buffer: a third of the LDS of a lockstep form, so three times the blocks
+// per CU, which is what this strided, bandwidth-bound pass needs.  All
+// global loads are issued before the first transform.
 template <typename R, int N, int E, int MODE = MA_ROWS_D>
 __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
     using C = cpx<R>;
@@ -88,18 +92,20 @@ __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
     const int total = a.ncols * G;
     typename F::Twiddles tw;
     tw.load(t, a.twx);
-    // the three fields go to buffers 3l, 3l+1, 3l+2 and are inverted in
-    // lockstep; all global loads are issued before the first LDS store
-    C* const b3[3] = {lds + (3 * l) * LE, lds + (3 * l + 1) * LE, lds + (3 * l + 2) * LE};
+    C* const buf = lds + l * LE;
     C qa0[MAXIT], qa1[MAXIT], qb0[MAXIT], qb1[MAXIT], qc0[MAXIT], qc1[MAXIT];
     ma_load<R, N, E, MAXIT>(a.Pxx, pitch, y0, G, total, qa0, qa1);
     ma_load<R, N, E, MAXIT>(a.Pyy, pitch, y0, G, total, qb0, qb1);
     ma_load<R, N, E, MAXIT>(a.Pxy, pitch, y0, G, total, qc0, qc1);
-    ma_scatter<R, N, E, MAXIT>(lds, G, total, qa0, qa1, 3);
-    ma_scatter<R, N, E, MAXIT>(lds + LE, G, total, qb0, qb1, 3);
-    ma_scatter<R, N, E, MAXIT>(lds + 2 * LE, G, total, qc0, qc1, 3);
     C w3[3][E];
-    F::template run_n_from_lds<+1, 3>(w3, b3, t, tw);
+    ma_scatter<R, N, E, MAXIT>(lds, G, total, qa0, qa1, 1);
+    F::template run_from_lds<+1>(w3[0], buf, t, tw);
+    __syncthreads();
+    ma_scatter<R, N, E, MAXIT>(lds, G, total, qb0, qb1, 1);
+    F::template run_from_lds<+1>(w3[1], buf, t, tw);
+    __syncthreads();
+    ma_scatter<R, N, E, MAXIT>(lds, G, total, qc0, qc1, 1);
+    F::template run_from_lds<+1>(w3[2], buf, t, tw);
     if constexpr (MODE == MA_ROWS_PHYS) {
         const size_t rA = (size_t)(y0 + l) * a.g.nx, rB = (size_t)(y0 + G + l) * a.g.nx;
 #pragma unroll
@@ -119,14 +125,12 @@ __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
         const C xx = w3[0][m], yy = w3[1][m], xy = w3[2][m];
         v[m] = C{xx.x * yy.x - xy.x * xy.x, xx.y * yy.y - xy.y * xy.y};
     }
-    C* buf = b3[0];
     F::template run<-1>(v, buf, t, tw);
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < E; ++m) buf[lpad(t + T * m)] = v[m];
     __syncthreads();
-    // line l's result sits in buffer 3l: store_split with a stride-3 view
-    store_split_stride<R, N, E>(lds, G, 3, a.ncols, a.Dx, a.Dx, pitch, y0, y0 + G);
+    store_split_stride<R, N, E>(lds, G, 1, a.ncols, a.Dx, a.Dx, pitch, y0, y0 + G);
 }
 
 // Adjoint row pass: λ (x-spectral) -> physical; the products Φyy λ, Φxx λ,

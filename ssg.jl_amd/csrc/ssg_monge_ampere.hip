@@ -1,0 +1,1003 @@
+// Monge-Ampère solve, coupled stepping and the discrete adjoints (host
+// side of ssg_ma.hpp); their C entry points.  Declarations: include/ssg.h.
+#include "ssg_internal.hpp"
+
+namespace ssg_impl {
+
+// ---------------------------------------------------------------------------
+// Monge-Ampère solve (FP64)
+// ---------------------------------------------------------------------------
+int ma_alloc(ssg_ctx* c) {
+    if (c->ma_ready) return SSG_OK;
+    if (c->nz < 3 || c->nz > 128)
+        return set_err(c, SSG_ERR_UNSUPPORTED, "the Monge-Ampere solve needs 3 <= nz <= 128 (nz = %d)", c->nz);
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    HIP_OK(c, hipMalloc(&c->ma_phi, c3 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->ma_dprev, c3 * sizeof(cpx<double>)));
+    for (int i = 0; i < 3; ++i) HIP_OK(c, hipMalloc(&c->ma_p[i], c3 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->ma_b2, (size_t)c->ncols_full * c->ny * sizeof(cpx<double>)));
+    if (c->precision == SSG_FP32_MIXED) {
+        HIP_OK(c, hipMalloc(&c->ma32_phi, c3 * sizeof(cpx<float>)));
+        HIP_OK(c, hipMalloc(&c->ma32_d, c3 * sizeof(cpx<float>)));
+    }
+    c->ma_partial_cap = c->ncols_full * c->nz + 64;
+    HIP_OK(c, hipMalloc(&c->ma_partial, (size_t)c->ma_partial_cap * 2 * sizeof(double)));
+    HIP_OK(c, hipHostMalloc(&c->ma_partial_host, (size_t)c->ma_partial_cap * 2 * sizeof(double), hipHostMallocDefault));
+    HIP_OK(c, hipMalloc(&c->bst, (size_t)c->nx * c->ny * sizeof(double)));
+    // stencil coefficients (uniform grid dz of the context)
+    const int nz = c->nz;
+    std::vector<double> dz(nz, c->lz / (nz - 1));
+    std::vector<double> cf(3 * nz + 6, 0.0);
+    for (int k = 1; k <= nz - 2; ++k) {  // d2dz2!, src/transforms.jl:812-819
+        const double dzm = dz[k - 1], dzp = dz[k];
+        cf[3 * k] = 2.0 / (dzm * (dzm + dzp));
+        cf[3 * k + 1] = -2.0 / (dzm * dzp);
+        cf[3 * k + 2] = 2.0 / (dzp * (dzm + dzp));
+    }
+    {  // apply_neumann_bottom!, src/poisson.jl:343-364
+        const double h1 = dz[0], h2 = dz[1], al = h1 + h2;
+        cf[3 * nz] = -(2 * h1 + h2) / (h1 * al);
+        cf[3 * nz + 1] = al / (h1 * h2);
+        cf[3 * nz + 2] = -h1 / (h2 * al);
+    }
+    {  // apply_neumann_surface!, src/poisson.jl:309-336
+        const double h1 = dz[nz - 2], h2 = dz[nz - 3], al = h1 + h2;
+        cf[3 * nz + 3] = (2 * h1 + h2) / (h1 * al);
+        cf[3 * nz + 4] = -al / (h1 * h2);
+        cf[3 * nz + 5] = h1 / (h2 * al);
+    }
+    HIP_OK(c, hipMalloc(&c->ma_coef, cf.size() * sizeof(double)));
+    HIP_OK(c, hipMemcpy(c->ma_coef, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->g3 = c->g;
+    c->g3.pitch = c->nz * c->ny;
+    c->ma_ready = true;
+    return SSG_OK;
+}
+
+void ma_free(ssg_ctx* c) {
+    (void)hipFree(c->ma_phi);
+    (void)hipFree(c->ma_dprev);
+    for (int i = 0; i < 3; ++i) (void)hipFree(c->ma_p[i]);
+    (void)hipFree(c->ma_b2);
+    (void)hipFree(c->ma32_phi);
+    (void)hipFree(c->ma32_d);
+    (void)hipFree(c->ma_coef);
+    (void)hipFree(c->ma_partial);
+    if (c->ma_partial_host) (void)hipHostFree(c->ma_partial_host);
+    (void)hipFree(c->bst);
+}
+
+// Φ̂ = L⁻¹(b̂ + ε D̂) per wavenumber (D̂ term omitted when use_d = 0)
+template <typename R, typename P>
+int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d, const cpx<R>* Dh, cpx<P>* Phi) {
+    MaTriArgs<R, P> a{};
+    a.Dh = Dh;
+    a.B2 = c->ma_b2;
+    a.Phi = Phi;
+    a.coef = c->ma_coef;
+    a.nz = c->nz;
+    a.ncols = c->ncols_full;
+    a.eps = p->eps;
+    a.body = p->variant == SSG_MA_BODY ? 1 : 0;
+    a.top = p->variant == SSG_MA_SURFACE ? 1 : 0;
+    a.use_d = use_d;
+    a.g = c->g;
+    const long total = (long)c->ncols_full * c->ny;
+    const int nblk = (int)((total + 127) / 128);
+    const double bytes = (double)total * (c->nz * (sizeof(cpx<R>) + sizeof(cpx<P>)) + sizeof(cpx<double>));
+    int slot, rc;
+    if ((rc = prof_begin(c, &slot))) return rc;
+    if (c->nz <= 8)
+        launch_on(c, slot, k_ma_tridiag<R, P, 8>, dim3(nblk), dim3(128), 0, a);
+    else if (c->nz <= 16)
+        launch_on(c, slot, k_ma_tridiag<R, P, 16>, dim3(nblk), dim3(128), 0, a);
+    else if (c->nz <= 32)
+        launch_on(c, slot, k_ma_tridiag<R, P, 32>, dim3(nblk), dim3(128), 0, a);
+    else if (c->nz <= 64)
+        launch_on(c, slot, k_ma_tridiag<R, P, 64>, dim3(nblk), dim3(128), 0, a);
+    else
+        launch_on(c, slot, k_ma_tridiag<R, P, 128>, dim3(nblk), dim3(128), 0, a);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    return prof_end(c, slot, 4, bytes);
+}
+
+// x-inverse of the three derivative planes, D = Φxx Φyy - Φxy², x-FFT of D
+// into planes[0]
+// elements per thread of k_ma_rows (its own choice: the tendency's row-pass
+// choice does not carry over to this strided, bandwidth-bound pass)
+template <typename R>
+int ma_ept_of(ssg_ctx* c) {
+    const int p = std::is_same<R, float>::value ? 1 : 0;
+    const int e = c->ma_ept[p];
+    if (e && ept_valid(c->nx, e)) return e;
+    return ept_valid(c->nx, 8) ? 8 : ept_default(c->nx);
+}
+
+template <typename R>
+int ma_rows(ssg_ctx* c, cpx<R>* const (&planes)[3]) {
+    PathBuffers<R>& B = bufs<R>(c);
+    const int ept = ma_ept_of<R>(c);
+    const int rows = c->nz * c->ny;
+    return with_size_ept(c->nx, ept, [&](auto tag, auto etag) -> int {
+        constexpr int N = decltype(tag)::value;
+        constexpr int E = decltype(etag)::value;
+        constexpr int T = N / E;
+        const int G = lines_per_block<R>(N, E, rows / 2, 1);
+        const int nblk = rows / (2 * G);
+        MaRowsArgs<R> a{planes[0], planes[1], planes[2], planes[0], {nullptr, nullptr, nullptr}, c->ncols_full, G, nblk,
+                        B.twx, c->g3};
+        const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<R>);
+        const double bytes = 4.0 * c->ncols_full * rows * sizeof(cpx<R>);
+        int slot, rc;
+        if ((rc = prof_begin(c, &slot))) return rc;
+        launch_on(c, slot, k_ma_rows<R, N, E>, dim3(nblk), dim3(G * T), lds, a);
+        c->launches++;
+        HIP_OK(c, hipGetLastError());
+        return prof_end(c, slot, 3, bytes);
+    });
+}
+
+template <typename A, typename B>
+int cpx_convert(ssg_ctx* c, const cpx<A>* in, cpx<B>* out, size_t n) {
+    hipLaunchKernelGGL((k_cpx_convert<A, B>), dim3(grid1d((long)n)), dim3(256), 0, c->stream, in, out, (long)n);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    return SSG_OK;
+}
+
+std::map<std::tuple<int, int, int, int, int>, int> g_ma_tune_cache;
+
+// Pick k_ma_rows' elements per thread for path R on this grid: two timed
+// launches per candidate after one untimed one, on the (allocated) planes.
+// SSG_MA_EPT=e forces a choice, SSG_TUNE=0 keeps the default.
+template <typename R>
+int ma_tune(ssg_ctx* c) {
+    const int p = std::is_same<R, float>::value ? 1 : 0;
+    if (c->ma_ept[p]) return SSG_OK;
+    if (const char* env = getenv("SSG_MA_EPT")) {
+        c->ma_ept[p] = atoi(env);
+        return SSG_OK;
+    }
+    const char* tv = getenv("SSG_TUNE");
+    if ((tv && tv[0] == '0') || (long)c->nx * c->ny * c->nz < 256L * 256L * 8L) {
+        c->ma_ept[p] = ma_ept_of<R>(c);
+        return SSG_OK;
+    }
+    const auto key = std::make_tuple(c->nx, c->ny, c->nz, p, c->device);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_ma_tune_cache.find(key);
+        if (it != g_ma_tune_cache.end()) {
+            c->ma_ept[p] = it->second;
+            return SSG_OK;
+        }
+    }
+    const bool prof_saved = c->prof;
+    c->prof = false;
+    cpx<R>* const pl[3] = {(cpx<R>*)c->ma_p[0], (cpx<R>*)c->ma_p[1], (cpx<R>*)c->ma_p[2]};
+    const int cands[3] = {4, 8, 16};
+    int best = 0, rc = SSG_OK;
+    double best_us = 1e30;
+    for (int ci = 0; ci < 3 && rc == SSG_OK; ++ci) {
+        if (!ept_valid(c->nx, cands[ci])) continue;
+        c->ma_ept[p] = cands[ci];
+        if ((rc = ma_rows<R>(c, pl))) break;
+        if (hipEventRecord(c->ev0, c->stream) != hipSuccess) {
+            rc = set_err(c, SSG_ERR_HIP, "tuning timing failed");
+            break;
+        }
+        for (int i = 0; i < 2 && rc == SSG_OK; ++i) rc = ma_rows<R>(c, pl);
+        if (rc) break;
+        float ms = 0;
+        if (hipEventRecord(c->ev1, c->stream) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess ||
+            hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
+            rc = set_err(c, SSG_ERR_HIP, "tuning timing failed");
+            break;
+        }
+        if (ms < best_us) {
+            best_us = ms;
+            best = cands[ci];
+        }
+    }
+    c->prof = prof_saved;
+    c->ma_ept[p] = best;
+    if (rc == SSG_OK && best) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_ma_tune_cache.emplace(key, best);
+    }
+    return rc;
+}
+
+// Residual evaluation of the iterate Φ̂ = L⁻¹(b̂ + ε D̂_prev): D̂ of Φ into
+// `dh` (which holds D̂_prev on entry) and the residual ε(D̂_prev - D̂) on
+// interior levels, reduced over blocks and read back.  `rabs` is
+// norm_field(r) (src/poisson.jl:986-995) via Parseval, `rrel` is
+// ‖r‖ / ‖b + εD‖.
+template <typename R>
+int ma_eval(ssg_ctx* c, const ssg_ma_params* p, const cpx<R>* phi, cpx<R>* const (&planes)[3], cpx<R>* dh,
+            double* rabs, double* rrel) {
+    const double scale = 1.0 / ((double)c->nx * c->ny);
+    const int lines = c->ncols_full * c->nz;
+    ColsArgs<R> a{};
+    a.inA = phi;
+    a.outA = planes[0];
+    a.outB = planes[1];
+    a.outC = planes[2];
+    a.scale = (R)scale;
+    int rc;
+    if ((rc = launch_cols_args<R, OP_MA_DERIV>(c, a, lines, c->nz))) return rc;
+    if ((rc = ma_rows<R>(c, planes))) return rc;
+    ColsArgs<R> b{};
+    b.inA = planes[0];
+    b.outA = dh;
+    b.B2 = c->ma_b2;
+    b.eps = p->eps;
+    b.body = p->variant == SSG_MA_BODY ? 1 : 0;
+    b.partial = c->ma_partial;
+    if ((rc = launch_cols_args<R, OP_MA_DIFF>(c, b, lines, c->nz))) return rc;
+    const int nb = c->ma_partial_used;
+    HIP_OK(c, hipMemcpyAsync(c->ma_partial_host, c->ma_partial, (size_t)nb * 2 * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    double r2 = 0, h2 = 0;
+    for (int i = 0; i < nb; ++i) {
+        r2 += c->ma_partial_host[2 * i];
+        h2 += c->ma_partial_host[2 * i + 1];
+    }
+    // Parseval: sum of r² over the grid = (1/(nx ny)) sum of |r̂|² per level
+    *rabs = std::sqrt(r2 * scale);
+    *rrel = h2 > 0 ? std::sqrt(r2 / h2) : std::sqrt(r2 * scale);
+    return SSG_OK;
+}
+
+// The FP32 phase stops once its residual estimate reaches this level (or
+// stops falling); below it FP32 rounding of D̂ dominates the residual.
+constexpr double kMaFp32Floor = 1e-6;
+
+// Picard / defect-correction iteration Φ̂ <- L⁻¹(b̂ + ε D̂(Φ)); the residual
+// of iterate n is ε(D̂ⁿ⁻¹ - D̂ⁿ) on interior levels, so every iteration
+// reports the reference residual of the Φ it holds.
+//
+// Mixed precision (SSG_FP32_MIXED contexts): the iteration first runs with
+// FP32 transforms, products and D̂ (the per-wavenumber solve stays FP64
+// arithmetic, rounded to FP32 on output).  The switch solves
+// Φ̂ = L⁻¹(b̂ + ε D̂₃₂) in FP64 and carries D̂₃₂ over as D̂_prev, so the first
+// FP64 evaluation yields the exact FP64 residual of that Φ; FP64 iterations
+// then continue to `tol`.  Convergence is only ever declared on an FP64
+// residual.
+int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    int rc0 = ma_tune<double>(c);
+    if (rc0 == SSG_OK && c->precision == SSG_FP32_MIXED) rc0 = ma_tune<float>(c);
+    if (rc0) return rc0;
+    const bool warm = p->warm && c->ma_have_d;
+    memset(d, 0, sizeof(*d));
+    int it = 0, rc;
+    double best = 1e300;
+    double ra = 0, rr = 0;
+    // record one evaluation; returns true when the iteration must stop
+    auto record = [&](bool fp64) -> bool {
+        d->final_residual = ra;
+        d->relative_residual = rr;
+        if (it < 64) d->history[it] = rr;
+        d->iterations = it;
+        ++it;
+        if (!std::isfinite(rr)) return true;
+        // divergence (loss of ellipticity): stop once the residual has grown
+        // a hundredfold over the best iterate
+        best = std::min(best, rr);
+        if (rr > 100.0 * best) return true;
+        if (fp64 && rr <= p->tol) {
+            d->converged = 1;
+            return true;
+        }
+        return it > p->maxiter;
+    };
+    if (c->precision == SSG_FP32_MIXED) {
+        cpx<float>* const pl[3] = {(cpx<float>*)c->ma_p[0], (cpx<float>*)c->ma_p[1], (cpx<float>*)c->ma_p[2]};
+        if (warm) {
+            if ((rc = cpx_convert<double, float>(c, c->ma_dprev, c->ma32_d, c3))) return rc;
+        } else {
+            HIP_OK(c, hipMemsetAsync(c->ma32_d, 0, c3 * sizeof(cpx<float>), c->stream));
+        }
+        if ((rc = ma_tridiag<float, float>(c, p, warm ? 1 : 0, c->ma32_d, c->ma32_phi))) return rc;
+        double prev = 1e300;
+        bool stop = false;
+        for (;;) {
+            if ((rc = ma_eval<float>(c, p, c->ma32_phi, pl, c->ma32_d, &ra, &rr))) return rc;
+            d->iterations_fp32++;
+            stop = record(false);
+            if (stop || rr <= std::max(p->tol, kMaFp32Floor) || rr > 0.7 * prev) break;
+            prev = rr;
+            if ((rc = ma_tridiag<float, float>(c, p, 1, c->ma32_d, c->ma32_phi))) return rc;
+        }
+        // switch to FP64; a maxiter stop still gets one FP64 evaluation, so
+        // the reported residual is always an FP64 one
+        if ((rc = cpx_convert<float, double>(c, c->ma32_d, c->ma_dprev, c3))) return rc;
+        if ((rc = ma_tridiag<float, double>(c, p, 1, c->ma32_d, c->ma_phi))) return rc;
+        if (stop && (!std::isfinite(rr) || rr > 100.0 * best)) {
+            c->ma_have_d = false;
+            return SSG_OK;
+        }
+    } else {
+        if (!warm) HIP_OK(c, hipMemsetAsync(c->ma_dprev, 0, c3 * sizeof(cpx<double>), c->stream));
+        if ((rc = ma_tridiag<double, double>(c, p, warm ? 1 : 0, c->ma_dprev, c->ma_phi))) return rc;
+    }
+    cpx<double>* const pl[3] = {c->ma_p[0], c->ma_p[1], c->ma_p[2]};
+    for (;;) {
+        if ((rc = ma_eval<double>(c, p, c->ma_phi, pl, c->ma_dprev, &ra, &rr))) return rc;
+        if (record(true)) break;
+        if ((rc = ma_tridiag<double, double>(c, p, 1, c->ma_dprev, c->ma_phi))) return rc;
+    }
+    c->ma_have_d = true;
+    return SSG_OK;
+}
+
+// φₛ = Φ[:,:,Nz] (src/poisson.jl:1119-1131) and u = -∂yφₛ, v = ∂xφₛ
+// (src/setICs.jl:161-192), taken from the top level of Φ̂
+int ma_surface_outputs(ssg_ctx* c) {
+    PathBuffers<double>& D = c->d;
+    const size_t lb = (size_t)c->ny * sizeof(cpx<double>);
+    HIP_OK(c, hipMemcpy2DAsync(D.Tp, lb, c->ma_phi + (size_t)(c->nz - 1) * c->ny, lb * c->nz, lb, c->ncols_full,
+                               hipMemcpyDeviceToDevice, c->stream));
+    const double scale = 1.0 / ((double)c->nx * c->ny);
+    ColsArgs<double> a{};
+    a.inA = D.Tp;
+    a.outA = D.Fx;
+    a.outB = D.Fy;
+    a.scale = scale;
+    int rc = launch_cols_args<double, OP_VEL_SPEC>(c, a, c->ncols_full, 1);
+    if (rc) return rc;
+    RowsInvArgs<double> w{};
+    w.P0 = D.Fx;
+    w.P1 = D.Fy;
+    w.ncols_in = c->ncols_full;
+    w.w_out = D.w;
+    if ((rc = launch_rows_inv<double, MODE_PLANES, EPI_STORE_W>(c, w))) return rc;
+    if ((rc = launch_cols<double, OP_INV>(c, D.Tp, nullptr, D.Tp, nullptr, c->ncols_full, scale))) return rc;
+    RowsInvArgs<double> f{};
+    f.P0 = D.Tp;
+    f.ncols_in = c->ncols_full;
+    f.out_r = c->phi;
+    if ((rc = launch_rows_inv<double, MODE_PAIR, EPI_STORE>(c, f))) return rc;
+    if (c->precision == SSG_FP32_MIXED) {
+        const long n = (long)c->nx * c->ny;
+        hipLaunchKernelGGL((k_w_convert<float>), dim3(grid1d(n)), dim3(256), 0, c->stream, D.w, c->f.w, n);
+        c->launches++;
+        HIP_OK(c, hipGetLastError());
+    }
+    return SSG_OK;
+}
+
+int ma_check_params(ssg_ctx* c, const ssg_ma_params* p) {
+    if (!p) return set_err(c, SSG_ERR_ARG, "null Monge-Ampere parameters");
+    if (!(p->eps >= 0) || !std::isfinite(p->eps)) return set_err(c, SSG_ERR_ARG, "eps must be finite and >= 0");
+    if (!(p->tol > 0)) return set_err(c, SSG_ERR_ARG, "tol must be > 0");
+    if (p->maxiter < 0) return set_err(c, SSG_ERR_ARG, "maxiter must be >= 0");
+    if (p->variant != SSG_MA_BODY && p->variant != SSG_MA_SURFACE)
+        return set_err(c, SSG_ERR_ARG, "unknown Monge-Ampere variant %d", p->variant);
+    return SSG_OK;
+}
+
+// full solve for the buoyancy `bs` (device, FP64)
+int ma_solve_dev(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_diag* d) {
+    int rc = ma_alloc(c);
+    if (rc) return rc;
+    if ((rc = rfft2_device(c, bs, c->ma_b2))) return rc;
+    if ((rc = ma_iterate(c, p, d))) return rc;
+    if ((rc = ma_surface_outputs(c))) return rc;
+    c->last_diag = *d;
+    return SSG_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of timestep! with frozen φₛ (FP64).  The step is linear in
+// b, so its transpose needs no trajectory: for RK3
+//   s3 = λ:              g3 = (3 dt/4) Tᵀ s3,  acc = λ + g3,  s2 = (2dt/3) g3
+//   s2:                  g2 = Tᵀ s2,           acc += g2,     s1 = (dt/4) λ + (dt/3) g2
+//   s1:                  g1 = Tᵀ s1,           λ ← acc + g1
+// with Tᵀ s = u Dx(M s) + v Dy(M s).  The filter E[σ] (real, even σ) is
+// self-adjoint and is applied to λ before the RK3 transpose of a filter step.
+// ---------------------------------------------------------------------------
+int adj_alloc(ssg_ctx* c) {
+    if (c->adj_lam) return SSG_OK;
+    const size_t n = (size_t)c->nx * c->ny;
+    HIP_OK(c, hipMalloc(&c->adj_lam, n * sizeof(double)));
+    HIP_OK(c, hipMalloc(&c->adj_acc, n * sizeof(double)));
+    HIP_OK(c, hipMalloc(&c->adj_s, n * sizeof(double)));
+    return SSG_OK;
+}
+
+template <int EPI>
+int adj_stage(ssg_ctx* c, const double* sin, double gscale, double dt) {
+    PathBuffers<double>& D = c->d;
+    int rc = launch_rows_fwd<double, double, SRC_PAIR>(c, sin, nullptr, D.Tp, nullptr, c->ncols_tend);
+    if (rc) return rc;
+    ColsArgs<double> a{};
+    a.inA = D.Tp;
+    a.outA = D.Fx;
+    a.outB = D.Fy;
+    a.scale = 1.0 / ((double)c->nx * c->ny);
+    if ((rc = launch_cols_args<double, OP_ADJ>(c, a, c->ncols_tend, 1))) return rc;
+    RowsInvArgs<double> r{};
+    r.P0 = D.Fx;
+    r.P1 = D.Fy;
+    r.ncols_in = c->ncols_tend;
+    r.w = D.w;
+    r.lam = c->adj_lam;
+    r.acc = c->adj_acc;
+    r.sbuf = c->adj_s;
+    r.c1 = dt / 3.0;
+    r.c2 = 2.0 * dt / 3.0;
+    r.c3 = dt / 4.0;
+    r.gscale = gscale;
+    return launch_rows_inv<double, MODE_PLANES, EPI>(c, r);
+}
+
+int adjoint_steps(ssg_ctx* c, const ssg_time_params* p, int64_t step_end, int64_t nsteps) {
+    const double dt = p->dt;
+    int rc;
+    for (int64_t n = step_end; n > step_end - nsteps; --n) {
+        // step n took b from step n-1 to n; the filter ran after it when n % F == 0
+        if (p->filter_freq > 0 && (n % p->filter_freq) == 0)
+            if ((rc = filter_b<double>(c, p->filter_strength, p->filter_order, p->filter_cutoff, 0, c->adj_lam)))
+                return rc;
+        if ((rc = adj_stage<EPI_ADJ3>(c, c->adj_lam, 3.0 * dt / 4.0, dt))) return rc;
+        if ((rc = adj_stage<EPI_ADJ2>(c, c->adj_s, 1.0, dt))) return rc;
+        if ((rc = adj_stage<EPI_ADJ1>(c, c->adj_s, 1.0, dt))) return rc;
+    }
+    return SSG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of the coupled step (Monge-Ampère solve at every stage),
+// FP64.  Stage s maps b_s -> Φ_s = MA(b_s) -> w_s = (-Dy φ_s, Dx φ_s) with
+// φ_s the top level -> k_s = T(w_s, b_s).  Its transpose, for an incoming
+// k̄:
+//   b̄_s = T_wᵀ k̄                      (frozen-velocity part)
+//       + Bᵀ Gᵀ (I - ε Nᵀ Gᵀ)⁻¹ φ̄     (through the linearised MA solve)
+//   φ̄ = Dy ū - Dx v̄,  (ū, v̄) = b_s (Dx M k̄, Dy M k̄)
+// where G is the per-wavenumber solve of k_ma_tridiag, N = D'(Φ_s) the
+// linearised D operator on interior rows and B places b on the right-hand
+// side (every interior row, or the top Neumann row for the surface
+// variant).  The bracket is solved by the transposed Picard iteration
+// F̄ <- Gᵀ(φ̄ + ε Nᵀ F̄), whose residual is ε Nᵀ(F̄ⁿ - F̄ⁿ⁻¹) -- the mirror
+// of the forward iteration.  The forward trajectory is recomputed from
+// checkpoints.
+// ---------------------------------------------------------------------------
+void adjc_free(ssg_ctx* c) {
+    for (int i = 0; i < 3; ++i) {
+        (void)hipFree(c->adjc_pd[i]);
+        (void)hipFree(c->adjc_bst[i]);
+        (void)hipFree(c->adjc_phis[i]);
+    }
+    (void)hipFree(c->adjc_fbar);
+    (void)hipFree(c->adjc_q);
+    (void)hipFree(c->adjc_lam);
+    (void)hipFree(c->adjc_phibar);
+    (void)hipFree(c->adjc_bbar);
+    (void)hipFree(c->adjc_wbar);
+    (void)hipFree(c->adjc_g);
+    (void)hipFree(c->adjc_k1);
+    (void)hipFree(c->adjc_tmp);
+}
+
+int adjc_alloc(ssg_ctx* c) {
+    int rc = ma_alloc(c);
+    if (rc) return rc;
+    if ((rc = adj_alloc(c))) return rc;
+    if (c->adjc_ready) return SSG_OK;
+    const size_t n2 = (size_t)c->nx * c->ny, r3 = n2 * c->nz;
+    const size_t c2 = (size_t)c->ncols_full * c->ny, c3 = c2 * c->nz;
+    for (int i = 0; i < 3; ++i) {
+        HIP_OK(c, hipMalloc(&c->adjc_pd[i], r3 * sizeof(double)));
+        HIP_OK(c, hipMalloc(&c->adjc_bst[i], n2 * sizeof(double)));
+        HIP_OK(c, hipMalloc(&c->adjc_phis[i], c3 * sizeof(cpx<double>)));
+    }
+    HIP_OK(c, hipMalloc(&c->adjc_fbar, c3 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->adjc_q, c3 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->adjc_lam, c3 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->adjc_phibar, c2 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->adjc_bbar, c2 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->adjc_wbar, n2 * sizeof(cpx<double>)));
+    HIP_OK(c, hipMalloc(&c->adjc_g, n2 * sizeof(double)));
+    HIP_OK(c, hipMalloc(&c->adjc_k1, n2 * sizeof(double)));
+    HIP_OK(c, hipMalloc(&c->adjc_tmp, n2 * sizeof(double)));
+    c->adjc_ready = true;
+    return SSG_OK;
+}
+
+int lincomb(ssg_ctx* c, double* out, double ca, const double* x, double cb, const double* y) {
+    const long n = (long)c->nx * c->ny;
+    hipLaunchKernelGGL(k_lincomb, dim3(grid1d(n)), dim3(256), 0, c->stream, out, ca, x, cb, y, n);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    return SSG_OK;
+}
+
+// inverse 2D transform of a [kx][ky] spectrum (unnormalized forward
+// convention) into a real field
+int irfft2_device(ssg_ctx* c, cpx<double>* spec, double* out) {
+    PathBuffers<double>& D = c->d;
+    const double scale = 1.0 / ((double)c->nx * c->ny);
+    int rc = launch_cols<double, OP_INV>(c, spec, nullptr, D.Tp, nullptr, c->ncols_full, scale);
+    if (rc) return rc;
+    RowsInvArgs<double> f{};
+    f.P0 = D.Tp;
+    f.ncols_in = c->ncols_full;
+    f.out_r = out;
+    return launch_rows_inv<double, MODE_PAIR, EPI_STORE>(c, f);
+}
+
+// Φxx, Φyy, Φxy of the current ma_phi in physical space (adjc_pd)
+int ma_phys_derivs(ssg_ctx* c) {
+    PathBuffers<double>& D = c->d;
+    const int lines = c->ncols_full * c->nz;
+    ColsArgs<double> a{};
+    a.inA = c->ma_phi;
+    a.outA = c->ma_p[0];
+    a.outB = c->ma_p[1];
+    a.outC = c->ma_p[2];
+    a.scale = 1.0 / ((double)c->nx * c->ny);
+    int rc = launch_cols_args<double, OP_MA_DERIV>(c, a, lines, c->nz);
+    if (rc) return rc;
+    const int ept = ma_ept_of<double>(c);
+    const int rows = c->nz * c->ny;
+    return with_size_ept(c->nx, ept, [&](auto tag, auto etag) -> int {
+        constexpr int N = decltype(tag)::value;
+        constexpr int E = decltype(etag)::value;
+        constexpr int T = N / E;
+        const int G = lines_per_block<double>(N, E, rows / 2, 1);
+        const int nblk = rows / (2 * G);
+        MaRowsArgs<double> m{c->ma_p[0], c->ma_p[1], c->ma_p[2], nullptr,
+                             {c->adjc_pd[0], c->adjc_pd[1], c->adjc_pd[2]}, c->ncols_full, G, nblk, D.twx, c->g3};
+        const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<double>);
+        hipLaunchKernelGGL((k_ma_rows<double, N, E, MA_ROWS_PHYS>), dim3(nblk), dim3(G * T), lds, c->stream, m);
+        c->launches++;
+        HIP_OK(c, hipGetLastError());
+        return SSG_OK;
+    });
+}
+
+int ma_adj_rows(ssg_ctx* c) {
+    PathBuffers<double>& D = c->d;
+    const int ept = ept_of<double>(c, 2, c->nx);
+    const int rows = c->nz * c->ny;
+    return with_size_ept(c->nx, ept, [&](auto tag, auto etag) -> int {
+        constexpr int N = decltype(tag)::value;
+        constexpr int E = decltype(etag)::value;
+        constexpr int T = N / E;
+        const int G = lines_per_block<double>(N, E, rows / 2, 3);
+        const int nblk = rows / (2 * G);
+        MaAdjRowsArgs<double> a{c->adjc_lam, c->adjc_pd[0], c->adjc_pd[1], c->adjc_pd[2],
+                                {c->ma_p[0], c->ma_p[1], c->ma_p[2]}, c->ncols_full, G, nblk, D.twx, c->g3};
+        const size_t lds = (size_t)G * 3 * lds_line_elems(N) * sizeof(cpx<double>);
+        hipLaunchKernelGGL((k_ma_adj_rows<double, N, E>), dim3(nblk), dim3(G * T), lds, c->stream, a);
+        c->launches++;
+        HIP_OK(c, hipGetLastError());
+        return SSG_OK;
+    });
+}
+
+int ma_tridiag_adj(ssg_ctx* c, const ssg_ma_params* p, int use_q) {
+    MaTriAdjArgs a{};
+    a.q = c->adjc_q;
+    a.phibar = c->adjc_phibar;
+    a.Fbar = c->adjc_fbar;
+    a.bbar = c->adjc_bbar;
+    a.coef = c->ma_coef;
+    a.nz = c->nz;
+    a.ncols = c->ncols_full;
+    a.eps = p->eps;
+    a.body = p->variant == SSG_MA_BODY ? 1 : 0;
+    a.top = p->variant == SSG_MA_SURFACE ? 1 : 0;
+    a.use_q = use_q;
+    a.g = c->g;
+    const long total = (long)c->ncols_full * c->ny;
+    const int nblk = (int)((total + 127) / 128);
+    if (c->nz <= 8)
+        hipLaunchKernelGGL((k_ma_tridiag_adj<8>), dim3(nblk), dim3(128), 0, c->stream, a);
+    else if (c->nz <= 16)
+        hipLaunchKernelGGL((k_ma_tridiag_adj<16>), dim3(nblk), dim3(128), 0, c->stream, a);
+    else if (c->nz <= 32)
+        hipLaunchKernelGGL((k_ma_tridiag_adj<32>), dim3(nblk), dim3(128), 0, c->stream, a);
+    else if (c->nz <= 64)
+        hipLaunchKernelGGL((k_ma_tridiag_adj<64>), dim3(nblk), dim3(128), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_ma_tridiag_adj<128>), dim3(nblk), dim3(128), 0, c->stream, a);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    return SSG_OK;
+}
+
+// Transposed linearised Monge-Ampère solve at the current ma_phi (whose
+// physical derivative fields are in adjc_pd): adjc_phibar = F(φ̄) in,
+// adjc_bbar = F(b̄) out.
+int ma_adjoint_solve(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
+    const double scale = 1.0 / ((double)c->nx * c->ny);
+    const int lines = c->ncols_full * c->nz;
+    const size_t c3 = (size_t)lines * c->ny;
+    memset(d, 0, sizeof(*d));
+    HIP_OK(c, hipMemsetAsync(c->adjc_q, 0, c3 * sizeof(cpx<double>), c->stream));
+    int rc = ma_tridiag_adj(c, p, 0);
+    if (rc) return rc;
+    double best = 1e300;
+    for (int it = 0;; ++it) {
+        ColsArgs<double> a{};
+        a.inA = c->adjc_fbar;
+        a.outA = c->adjc_lam;
+        a.scale = scale;
+        if ((rc = launch_cols_args<double, OP_INV>(c, a, lines, c->nz))) return rc;
+        if ((rc = ma_adj_rows(c))) return rc;
+        ColsArgs<double> b{};
+        b.inA = c->ma_p[0];
+        b.inB = c->ma_p[1];
+        b.inC = c->ma_p[2];
+        b.outA = c->adjc_q;
+        b.B2 = c->adjc_phibar;
+        b.eps = p->eps;
+        b.partial = c->ma_partial;
+        if ((rc = launch_cols_args<double, OP_MA_ADJ>(c, b, lines, c->nz))) return rc;
+        const int nb = c->ma_partial_used;
+        HIP_OK(c, hipMemcpyAsync(c->ma_partial_host, c->ma_partial, (size_t)nb * 2 * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(c, hipStreamSynchronize(c->stream));
+        double r2 = 0, h2 = 0;
+        for (int i = 0; i < nb; ++i) {
+            r2 += c->ma_partial_host[2 * i];
+            h2 += c->ma_partial_host[2 * i + 1];
+        }
+        d->final_residual = std::sqrt(r2 * scale);
+        d->relative_residual = h2 > 0 ? std::sqrt(r2 / h2) : 0.0;
+        if (it < 64) d->history[it] = d->relative_residual;
+        d->iterations = it;
+        if (!std::isfinite(d->relative_residual)) break;
+        best = std::min(best, d->relative_residual);
+        if (d->relative_residual > 100.0 * best) break;
+        if (d->relative_residual <= p->tol) {
+            d->converged = 1;
+            break;
+        }
+        if (it >= p->maxiter) break;
+        if ((rc = ma_tridiag_adj(c, p, 1))) return rc;
+    }
+    c->adjc_diag = *d;
+    if (!d->converged)
+        return set_err(c, SSG_ERR_NOT_CONVERGED, "adjoint Monge-Ampere solve did not converge (rel. residual %.3e)",
+                       d->relative_residual);
+    return SSG_OK;
+}
+
+// One coupled RK3 step in FP64 from c->b, recording the stage buoyancies
+// and their Monge-Ampère solutions; c->b ends at the next step's state.
+int coupled_step_record(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* mp, int64_t step_after) {
+    const double dt = p->dt;
+    const size_t n2 = (size_t)c->nx * c->ny;
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    ssg_ma_diag d{};
+    int rc;
+    HIP_OK(c, hipMemcpyAsync(c->adjc_bst[0], c->b, n2 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    const int epi_next[3] = {EPI_RK1, EPI_RK2, EPI_RK3};
+    for (int st = 0; st < 3; ++st) {
+        if ((rc = ma_solve_dev(c, c->adjc_bst[st], mp, &d))) return rc;
+        if (!d.converged)
+            return set_err(c, SSG_ERR_NOT_CONVERGED, "Monge-Ampere solve did not converge (rel. residual %.3e)",
+                           d.relative_residual);
+        HIP_OK(c, hipMemcpyAsync(c->adjc_phis[st], c->ma_phi, c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice,
+                                 c->stream));
+        if ((rc = flux_forward<double>(c, c->ncols_tend, c->adjc_bst[st]))) return rc;
+        if ((rc = tendency_cols<double>(c, 0))) return rc;
+        if (epi_next[st] == EPI_RK1)
+            rc = stage_inverse<double, EPI_RK1>(c, dt / 3.0, 0, 0, c->adjc_bst[1]);
+        else if (epi_next[st] == EPI_RK2)
+            rc = stage_inverse<double, EPI_RK2>(c, 2.0 * dt / 3.0, 0, 0, c->adjc_bst[2]);
+        else
+            rc = stage_inverse<double, EPI_RK3>(c, dt / 4.0, 0, 0);
+        if (rc) return rc;
+    }
+    if (p->filter_freq > 0 && (step_after % p->filter_freq) == 0)
+        return filter_b<double>(c, p->filter_strength, p->filter_order, p->filter_cutoff, 0);
+    return SSG_OK;
+}
+
+// b̄ of one stage into adjc_g, for the incoming tendency adjoint `kbar`
+int adjc_stage(ssg_ctx* c, const ssg_ma_params* mp, const double* kbar, int st) {
+    PathBuffers<double>& D = c->d;
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    const double scale = 1.0 / ((double)c->nx * c->ny);
+    // the stage's Monge-Ampère solution, its velocities and derivative fields
+    HIP_OK(c, hipMemcpyAsync(c->ma_phi, c->adjc_phis[st], c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice,
+                             c->stream));
+    int rc = ma_surface_outputs(c);
+    if (rc) return rc;
+    if ((rc = ma_phys_derivs(c))) return rc;
+    // frozen-velocity transpose and the velocity adjoint (ū, v̄)
+    if ((rc = launch_rows_fwd<double, double, SRC_PAIR>(c, kbar, nullptr, D.Tp, nullptr, c->ncols_tend))) return rc;
+    ColsArgs<double> a{};
+    a.inA = D.Tp;
+    a.outA = D.Fx;
+    a.outB = D.Fy;
+    a.scale = scale;
+    if ((rc = launch_cols_args<double, OP_ADJ>(c, a, c->ncols_tend, 1))) return rc;
+    RowsInvArgs<double> r{};
+    r.P0 = D.Fx;
+    r.P1 = D.Fy;
+    r.ncols_in = c->ncols_tend;
+    r.w = D.w;
+    r.b = c->adjc_bst[st];
+    r.out_r = c->adjc_g;
+    r.w_out = c->adjc_wbar;
+    r.gscale = 1.0;
+    if ((rc = launch_rows_inv<double, MODE_PLANES, EPI_ADJW>(c, r))) return rc;
+    // F(φ̄) = F(Dy ū - Dx v̄)
+    if ((rc = launch_rows_fwd<double, double, SRC_W>(c, nullptr, c->adjc_wbar, D.Fx, D.Fy, c->ncols_full))) return rc;
+    ColsArgs<double> ph{};
+    ph.inA = D.Fx;
+    ph.inB = D.Fy;
+    ph.outA = c->adjc_phibar;
+    ph.scale = 1.0;
+    if ((rc = launch_cols_args<double, OP_PHIBAR>(c, ph, c->ncols_full, 1))) return rc;
+    // through the linearised Monge-Ampère solve
+    ssg_ma_diag d{};
+    if ((rc = ma_adjoint_solve(c, mp, &d))) return rc;
+    if ((rc = irfft2_device(c, c->adjc_bbar, c->adjc_tmp))) return rc;
+    return lincomb(c, c->adjc_g, 1.0, c->adjc_g, 1.0, c->adjc_tmp);
+}
+
+// Reverse sweep over steps [step0 + 1, step0 + nsteps] with the forward
+// states b(step0 + j) provided by `state_at(j)` (which leaves it in c->b).
+template <typename StateAt>
+int adjoint_coupled_sweep(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* mp, int64_t step0,
+                          int64_t nsteps, StateAt&& state_at) {
+    const double dt = p->dt;
+    int rc;
+    for (int64_t j = nsteps; j >= 1; --j) {
+        const int64_t n = step0 + j;  // step n took b(n-1) to b(n)
+        if (p->filter_freq > 0 && (n % p->filter_freq) == 0)
+            if ((rc = filter_b<double>(c, p->filter_strength, p->filter_order, p->filter_cutoff, 0, c->adj_lam)))
+                return rc;
+        if ((rc = state_at(j - 1))) return rc;
+        if ((rc = coupled_step_record(c, p, mp, n))) return rc;
+        // λ̄ of b(n-1): acc = λ + Σ stage terms
+        if ((rc = lincomb(c, c->adj_acc, 1.0, c->adj_lam, 0.0, nullptr))) return rc;
+        if ((rc = lincomb(c, c->adj_s, 0.75 * dt, c->adj_lam, 0.0, nullptr))) return rc;      // k̄3
+        if ((rc = lincomb(c, c->adjc_k1, 0.25 * dt, c->adj_lam, 0.0, nullptr))) return rc;    // k̄1
+        if ((rc = adjc_stage(c, mp, c->adj_s, 2))) return rc;                                 // b̄2
+        if ((rc = lincomb(c, c->adj_acc, 1.0, c->adj_acc, 1.0, c->adjc_g))) return rc;
+        if ((rc = lincomb(c, c->adj_s, 2.0 * dt / 3.0, c->adjc_g, 0.0, nullptr))) return rc;  // k̄2
+        if ((rc = adjc_stage(c, mp, c->adj_s, 1))) return rc;                                 // b̄1
+        if ((rc = lincomb(c, c->adj_acc, 1.0, c->adj_acc, 1.0, c->adjc_g))) return rc;
+        if ((rc = lincomb(c, c->adjc_k1, 1.0, c->adjc_k1, dt / 3.0, c->adjc_g))) return rc;
+        if ((rc = adjc_stage(c, mp, c->adjc_k1, 0))) return rc;                               // b̄0 part
+        if ((rc = lincomb(c, c->adj_lam, 1.0, c->adj_acc, 1.0, c->adjc_g))) return rc;
+    }
+    return SSG_OK;
+}
+
+// checkpointed adjoint of nsteps coupled steps from the context's current b
+// (at step number step0): checkpoints every `every` steps on the forward
+// pass, then each segment is recomputed from its checkpoint, storing the
+// per-step states, and swept in reverse.
+int adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* mp, int64_t step0, int64_t nsteps,
+                    int every) {
+    const size_t n2 = (size_t)c->nx * c->ny, bytes = n2 * sizeof(double);
+    const int64_t nck = (nsteps + every - 1) / every;
+    double* ck = nullptr;
+    double* seg = nullptr;
+    HIP_OK(c, hipMalloc(&ck, (size_t)nck * bytes));
+    if (hipMalloc(&seg, (size_t)every * bytes) != hipSuccess) {
+        (void)hipFree(ck);
+        return set_err(c, SSG_ERR_HIP, "checkpoint allocation failed");
+    }
+    auto body = [&]() -> int {
+        int rc;
+        // forward pass with checkpoints; c->b is restored at the end
+        for (int64_t j = 0; j < nsteps; ++j) {
+            if (j % every == 0)
+                HIP_OK(c, hipMemcpyAsync(ck + (size_t)(j / every) * n2, c->b, bytes, hipMemcpyDeviceToDevice,
+                                         c->stream));
+            if ((rc = coupled_step_record(c, p, mp, step0 + j + 1))) return rc;
+        }
+        for (int64_t sg = nck - 1; sg >= 0; --sg) {
+            const int64_t j0 = sg * every;
+            const int64_t len = std::min<int64_t>(every, nsteps - j0);
+            // recompute the segment's per-step states from its checkpoint
+            HIP_OK(c, hipMemcpyAsync(c->b, ck + (size_t)sg * n2, bytes, hipMemcpyDeviceToDevice, c->stream));
+            for (int64_t k = 0; k < len; ++k) {
+                HIP_OK(c, hipMemcpyAsync(seg + (size_t)k * n2, c->b, bytes, hipMemcpyDeviceToDevice, c->stream));
+                if (k + 1 < len && (rc = coupled_step_record(c, p, mp, step0 + j0 + k + 1))) return rc;
+            }
+            auto state_at = [&](int64_t jj) -> int {
+                HIP_OK(c, hipMemcpyAsync(c->b, seg + (size_t)jj * n2, bytes, hipMemcpyDeviceToDevice, c->stream));
+                return SSG_OK;
+            };
+            if ((rc = adjoint_coupled_sweep(c, p, mp, step0 + j0, len, state_at))) return rc;
+        }
+        // leave b at the start state
+        HIP_OK(c, hipMemcpyAsync(c->b, ck, bytes, hipMemcpyDeviceToDevice, c->stream));
+        return SSG_OK;
+    };
+    const int rc = body();
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(ck);
+    (void)hipFree(seg);
+    return rc;
+}
+
+}  // namespace ssg_impl
+
+extern "C" {
+
+int ssg_set_b(ssg_ctx* c, const double* host, int64_t n, const ssg_ma_params* p, ssg_ma_diag* d) {
+    if (!c) return SSG_ERR_ARG;
+    if (!host) return set_err(c, SSG_ERR_ARG, "null pointer");
+    if (n != (int64_t)c->nx * c->ny)
+        return set_err(c, SSG_ERR_ARG, "field size %lld != nx*ny", (long long)n);
+    int rc;
+    if (p && (rc = ma_check_params(c, p))) return rc;
+    if ((rc = ssg_set_field(c, SSG_FIELD_B, host, n))) return rc;
+    // zero mean and 2/3 dealias in spectral space (src/setICs.jl:30-52)
+    PathBuffers<double>& D = c->d;
+    if ((rc = launch_rows_fwd<double, double, SRC_PAIR>(c, c->b, nullptr, D.Tp, nullptr, c->ncols_full))) return rc;
+    if ((rc = launch_cols<double, OP_SETB>(c, D.Tp, nullptr, D.Tp, nullptr, c->ncols_full,
+                                           1.0 / ((double)c->nx * c->ny))))
+        return rc;
+    RowsInvArgs<double> a{};
+    a.P0 = D.Tp;
+    a.ncols_in = c->ncols_full;
+    a.b = c->b;
+    if ((rc = launch_rows_inv<double, MODE_PAIR, EPI_SET>(c, a))) return rc;
+    if (!p) {
+        HIP_OK(c, hipStreamSynchronize(c->stream));
+        return SSG_OK;
+    }
+    // the Monge-Ampère solve for φₛ and the surface velocities
+    // (src/setICs.jl:55-62)
+    ssg_ma_diag dd{};
+    HIP_OK(c, hipEventRecord(c->ev0, c->stream));
+    rc = ma_solve_dev(c, c->b, p, &dd);
+    HIP_OK(c, hipEventRecord(c->ev1, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIP_OK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    dd.solve_ms = ms;
+    c->last_diag = dd;
+    if (d) *d = dd;
+    if (rc) return rc;
+    if (!dd.converged)
+        return set_err(c, SSG_ERR_NOT_CONVERGED, "Monge-Ampere solve did not converge (rel. residual %.3e)",
+                       dd.relative_residual);
+    return SSG_OK;
+}
+
+int ssg_ma_solve(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
+    if (!c) return SSG_ERR_ARG;
+    int rc = ma_check_params(c, p);
+    if (rc) return rc;
+    HIP_OK(c, hipSetDevice(c->device));
+    ssg_ma_diag local{};
+    if (!d) d = &local;
+    HIP_OK(c, hipEventRecord(c->ev0, c->stream));
+    if ((rc = ma_solve_dev(c, c->b, p, d))) return rc;
+    HIP_OK(c, hipEventRecord(c->ev1, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIP_OK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    d->solve_ms = ms;
+    c->last_diag = *d;
+    if (!d->converged)
+        return set_err(c, SSG_ERR_NOT_CONVERGED, "not converged after %d iterations (rel. residual %.3e)", d->iterations,
+                       d->relative_residual);
+    return SSG_OK;
+}
+
+int ssg_get_phi3d(ssg_ctx* c, double* host, int64_t n) {
+    if (!c) return SSG_ERR_ARG;
+    if (!host) return set_err(c, SSG_ERR_ARG, "null pointer");
+    const int64_t need = (int64_t)c->nx * c->ny * c->nz;
+    if (n != need) return set_err(c, SSG_ERR_ARG, "size %lld != nx*ny*nz = %lld", (long long)n, (long long)need);
+    if (!c->ma_ready) return set_err(c, SSG_ERR_ARG, "no Monge-Ampere solution yet");
+    HIP_OK(c, hipSetDevice(c->device));
+    const int lines = c->ncols_full * c->nz;
+    ColsArgs<double> a{};
+    a.inA = c->ma_phi;
+    a.outA = c->ma_p[0];
+    a.scale = 1.0 / ((double)c->nx * c->ny);
+    int rc = launch_cols_args<double, OP_INV>(c, a, lines, c->nz);
+    if (rc) return rc;
+    RowsInvArgs<double> r{};
+    r.P0 = c->ma_p[0];
+    r.ncols_in = c->ncols_full;
+    r.out_r = reinterpret_cast<double*>(c->ma_p[1]);
+    if ((rc = launch_rows_inv<double, MODE_PAIR, EPI_STORE>(c, r, c->nz * c->ny, &c->g3))) return rc;
+    HIP_OK(c, hipMemcpyAsync(host, c->ma_p[1], (size_t)need * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+int ssg_set_coupling(ssg_ctx* c, const ssg_ma_params* p) {
+    if (!c) return SSG_ERR_ARG;
+    if (!p) {
+        c->coupled = false;
+        return SSG_OK;
+    }
+    int rc = ma_check_params(c, p);
+    if (rc) return rc;
+    HIP_OK(c, hipSetDevice(c->device));
+    if ((rc = ma_alloc(c))) return rc;
+    c->ma_par = *p;
+    c->coupled = true;
+    return SSG_OK;
+}
+
+int ssg_adjoint(ssg_ctx* c, const ssg_time_params* p, int64_t step_end, int64_t nsteps, const double* lam_in,
+                double* lam_out) {
+    if (!c) return SSG_ERR_ARG;
+    if (!p || !lam_in || !lam_out) return set_err(c, SSG_ERR_ARG, "null pointer");
+    if (p->scheme != SSG_RK3) return set_err(c, SSG_ERR_UNSUPPORTED, "the adjoint covers the RK3 scheme");
+    if (c->coupled) return set_err(c, SSG_ERR_UNSUPPORTED, "the adjoint covers the frozen-φₛ step");
+    if (nsteps < 0 || step_end < nsteps) return set_err(c, SSG_ERR_ARG, "need 0 <= nsteps <= step_end");
+    if (!(p->dt > 0)) return set_err(c, SSG_ERR_ARG, "dt must be > 0");
+    HIP_OK(c, hipSetDevice(c->device));
+    int rc = adj_alloc(c);
+    if (rc) return rc;
+    const size_t bytes = (size_t)c->nx * c->ny * sizeof(double);
+    HIP_OK(c, hipMemcpyAsync(c->adj_lam, lam_in, bytes, hipMemcpyHostToDevice, c->stream));
+    if ((rc = adjoint_steps(c, p, step_end, nsteps))) return rc;
+    HIP_OK(c, hipMemcpyAsync(lam_out, c->adj_lam, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+int ssg_adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* mp, int64_t step_start,
+                        int64_t nsteps, int32_t checkpoint_every, const double* lam_in, double* lam_out) {
+    if (!c) return SSG_ERR_ARG;
+    if (!p || !mp || !lam_in || !lam_out) return set_err(c, SSG_ERR_ARG, "null pointer");
+    if (p->scheme != SSG_RK3) return set_err(c, SSG_ERR_UNSUPPORTED, "the adjoint covers the RK3 scheme");
+    if (nsteps < 0 || step_start < 0) return set_err(c, SSG_ERR_ARG, "need nsteps >= 0 and step_start >= 0");
+    if (checkpoint_every < 1) return set_err(c, SSG_ERR_ARG, "checkpoint_every must be >= 1");
+    if (!(p->dt > 0)) return set_err(c, SSG_ERR_ARG, "dt must be > 0");
+    int rc = ma_check_params(c, mp);
+    if (rc) return rc;
+    HIP_OK(c, hipSetDevice(c->device));
+    if ((rc = adjc_alloc(c))) return rc;
+    const size_t bytes = (size_t)c->nx * c->ny * sizeof(double);
+    HIP_OK(c, hipMemcpyAsync(c->adj_lam, lam_in, bytes, hipMemcpyHostToDevice, c->stream));
+    if (nsteps > 0 && (rc = adjoint_coupled(c, p, mp, step_start, nsteps, checkpoint_every))) return rc;
+    HIP_OK(c, hipMemcpyAsync(lam_out, c->adj_lam, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+int ssg_ma_adjoint(ssg_ctx* c, const ssg_ma_params* mp, const double* phibar, double* bbar, ssg_ma_diag* diag) {
+    if (!c) return SSG_ERR_ARG;
+    if (!mp || !phibar || !bbar) return set_err(c, SSG_ERR_ARG, "null pointer");
+    int rc = ma_check_params(c, mp);
+    if (rc) return rc;
+    if (!c->ma_ready || !c->ma_have_d)
+        return set_err(c, SSG_ERR_ARG, "no Monge-Ampere solution to linearise about (call ssg_ma_solve first)");
+    HIP_OK(c, hipSetDevice(c->device));
+    if ((rc = adjc_alloc(c))) return rc;
+    const size_t bytes = (size_t)c->nx * c->ny * sizeof(double);
+    HIP_OK(c, hipMemcpyAsync(c->adjc_tmp, phibar, bytes, hipMemcpyHostToDevice, c->stream));
+    if ((rc = rfft2_device(c, c->adjc_tmp, c->adjc_phibar))) return rc;
+    if ((rc = ma_phys_derivs(c))) return rc;
+    ssg_ma_diag d{};
+    rc = ma_adjoint_solve(c, mp, &d);
+    if (diag) *diag = d;
+    if (rc) return rc;
+    if ((rc = irfft2_device(c, c->adjc_bbar, c->adjc_tmp))) return rc;
+    HIP_OK(c, hipMemcpyAsync(bbar, c->adjc_tmp, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+int ssg_last_ma_diag(ssg_ctx* c, ssg_ma_diag* d) {
+    if (!c || !d) return SSG_ERR_ARG;
+    *d = c->last_diag;
+    return SSG_OK;
+}
+
+}  // extern "C"

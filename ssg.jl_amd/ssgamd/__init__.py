@@ -52,7 +52,8 @@ EXPORTED_SYMBOLS = (
     "ssg_profile", "ssg_profile_read", "ssg_tuning", "ssg_ma_solve", "ssg_get_phi3d", "ssg_set_coupling",
     "ssg_last_ma_diag", "ssg_copy_probe", "ssg_adjoint", "ssg_ensemble_create", "ssg_ensemble_destroy",
     "ssg_ensemble_member", "ssg_ensemble_last_error", "ssg_ensemble_timestep", "ssg_ensemble_tuning",
-    "ssg_ma_adjoint", "ssg_adjoint_coupled",
+    "ssg_ma_adjoint", "ssg_adjoint_coupled", "ssg_set_b", "ssg_cfl", "ssg_state_size", "ssg_save_state",
+    "ssg_load_state",
 )
 
 MA_BODY = 0      # as wired: b on every level, surface slope 0
@@ -163,6 +164,11 @@ def load_library(path: str | None = None):
     lib.ssg_ensemble_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), _dp]
     lib.ssg_profile_read.argtypes = [vp, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), _dp]
     lib.ssg_ma_adjoint.argtypes = [vp, ctypes.POINTER(_MaParams), _dp, _dp, ctypes.POINTER(_MaDiag)]
+    lib.ssg_set_b.argtypes = [vp, _dp, ctypes.c_int64, ctypes.POINTER(_MaParams), ctypes.POINTER(_MaDiag)]
+    lib.ssg_cfl.argtypes = [vp, ctypes.c_double, _dp, _dp, _dp]
+    lib.ssg_state_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+    lib.ssg_save_state.argtypes = [vp, ctypes.POINTER(_TimeState), vp, ctypes.c_size_t]
+    lib.ssg_load_state.argtypes = [vp, vp, ctypes.c_size_t, ctypes.POINTER(_TimeState)]
     lib.ssg_adjoint_coupled.argtypes = [vp, ctypes.POINTER(_TimeParams), ctypes.POINTER(_MaParams), ctypes.c_int64,
                                         ctypes.c_int64, ctypes.c_int32, _dp, _dp]
     for n in EXPORTED_SYMBOLS:
@@ -194,6 +200,7 @@ class Domain:
         self.Nx, self.Ny, self.Nz = int(nx), int(ny), int(nz)
         self.Lx, self.Ly, self.Lz = float(Lx), float(Ly), float(Lz)
         self.precision = int(precision)
+        self.coupled = False
         self._owned = _borrowed is None
         if _borrowed is not None:
             h = ctypes.c_void_p(_borrowed)
@@ -340,12 +347,10 @@ class TimeParams:
     ab2_reference_aliasing: bool = False
     check_finite: bool = False
 
-    def _c(self) -> _TimeParams:
-        if self.adaptive_dt:
-            raise SSGError(SSG_ERR_UNSUPPORTED, "adaptive_dt is not supported on the GPU path")
-        return _TimeParams(float(self.dt), int(self.scheme), int(self.filter_freq), float(self.filter_strength),
-                           int(self.filter_order), float(self.filter_cutoff), int(bool(self.ab2_reference_aliasing)),
-                           int(bool(self.check_finite)))
+    def _c(self, dt: float | None = None) -> _TimeParams:
+        return _TimeParams(float(self.dt if dt is None else dt), int(self.scheme), int(self.filter_freq),
+                           float(self.filter_strength), int(self.filter_order), float(self.filter_cutoff),
+                           int(bool(self.ab2_reference_aliasing)), int(bool(self.check_finite)))
 
 
 class TimeState:
@@ -354,23 +359,67 @@ class TimeState:
 
     def __init__(self, initial_time: float = 0.0, fields: Fields | None = None):
         self._c = _TimeState(float(initial_time), 0, 0.0)
+        self.cfl_max = 0.0
 
     @property
     def t(self):
         return self._c.t
 
+    @t.setter
+    def t(self, v):
+        self._c.t = float(v)
+
     @property
     def step(self):
         return self._c.step
+
+    @step.setter
+    def step(self, v):
+        self._c.step = int(v)
 
     @property
     def dt_actual(self):
         return self._c.dt_actual
 
+    @dt_actual.setter
+    def dt_actual(self, v):
+        self._c.dt_actual = float(v)
+
+
+def adaptive_timestep(fields: Fields, domain: Domain, params: TimeParams, state: TimeState) -> float:
+    """``adaptive_timestep`` (src/timestep.jl:122-147): params.dt scaled by the
+    CFL number of the current surface velocities."""
+    if not params.adaptive_dt:
+        return params.dt
+    cfl = compute_cfl_number(fields, domain, params.dt)
+    state.cfl_max = cfl
+    if cfl > params.cfl_safety:
+        return max(params.dt * params.cfl_safety / cfl, params.min_dt)
+    if cfl < 0.5 * params.cfl_safety:
+        return min(params.dt * 1.1, params.max_dt)
+    return params.dt
+
 
 def timestep(fields: Fields, domain: Domain, params: TimeParams, state: TimeState, nsteps: int = 1) -> float:
-    """``timestep!`` (src/timestep.jl:267-285), ``nsteps`` times.  Returns dt."""
-    p = params._c()
+    """``timestep!`` (src/timestep.jl:267-285), ``nsteps`` times.  Returns dt.
+
+    With ``params.adaptive_dt`` each step's dt comes from
+    :func:`adaptive_timestep`; with φₛ frozen the velocities, and so dt, do
+    not change within a call, so the call stays one batch on the device."""
+    if params.adaptive_dt and nsteps > 0:
+        if domain.coupled:
+            for _ in range(int(nsteps)):
+                timestep(fields, domain, TimeParams(adaptive_timestep(fields, domain, params, state),
+                                                    scheme=params.scheme, filter_freq=params.filter_freq,
+                                                    filter_strength=params.filter_strength,
+                                                    filter_order=params.filter_order,
+                                                    filter_cutoff=params.filter_cutoff,
+                                                    ab2_reference_aliasing=params.ab2_reference_aliasing,
+                                                    check_finite=params.check_finite), state, 1)
+            return state.dt_actual
+        p = params._c(adaptive_timestep(fields, domain, params, state))
+    else:
+        p = params._c()
     domain.check(domain.lib.ssg_timestep(domain.handle, ctypes.byref(p), ctypes.byref(state._c), int(nsteps)))
     return state.dt_actual
 
@@ -453,9 +502,75 @@ def set_coupling(domain: Domain, *, eps: float | None = 0.1, tol: float = 1e-10,
     (``eps=None`` restores the reference's frozen φₛ)."""
     if eps is None:
         domain.check(domain.lib.ssg_set_coupling(domain.handle, None))
+        domain.coupled = False
         return
     p = _ma_params(eps, tol, maxiter, variant, warm)
     domain.check(domain.lib.ssg_set_coupling(domain.handle, ctypes.byref(p)))
+    domain.coupled = True
+
+
+def set_b(fields: Fields, domain: Domain, b, *, eps: float | None = 0.1, tol: float = 1e-10, maxiter: int = 20,
+          variant: int = MA_BODY) -> dict | None:
+    """``set_b!`` (src/setICs.jl:21-88): the k = 0 mode of ``b`` removed and the
+    2/3 mask applied, then (``eps`` not None) the Monge-Ampère solve for φₛ and
+    the surface velocities.  Returns the solve's diagnostics (None without a
+    solve); ``converged`` is False, without an exception, when ``maxiter`` is
+    reached, as in the reference."""
+    b = _f64(b, (domain.Ny, domain.Nx))
+    d = _MaDiag()
+    p = None if eps is None else _ma_params(eps, tol, maxiter, variant, False)
+    rc = domain.lib.ssg_set_b(domain.handle, b.ctypes.data_as(_dp), b.size,
+                              None if p is None else ctypes.byref(p), ctypes.byref(d))
+    if rc not in (SSG_OK, SSG_ERR_NOT_CONVERGED):
+        domain.check(rc)
+    if p is None:
+        return None
+    out = d.as_dict()
+    out["epsilon_parameter"] = float(eps)
+    return out
+
+
+def compute_cfl_number(fields: Fields, domain: Domain, dt: float) -> float:
+    """``compute_cfl_number`` (src/timestep.jl:91-117) from the surface
+    velocities: max(max|u| dt/dx, max|v| dt/dy)."""
+    cfl, um, vm = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    domain.check(domain.lib.ssg_cfl(domain.handle, float(dt), ctypes.byref(cfl), ctypes.byref(um), ctypes.byref(vm)))
+    return cfl.value
+
+
+def save_state(domain: Domain, state: "TimeState") -> bytes:
+    """The step state (b, φₛ, the AB2 history, t and step) as bytes; loading
+    it into a context of the same grid and precision resumes bitwise for the
+    frozen-φₛ step."""
+    n = ctypes.c_size_t()
+    domain.check(domain.lib.ssg_state_size(domain.handle, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value)
+    domain.check(domain.lib.ssg_save_state(domain.handle, ctypes.byref(state._c), buf, n.value))
+    return buf.raw
+
+
+def load_state(domain: Domain, blob: bytes, state: "TimeState") -> None:
+    buf = ctypes.create_string_buffer(blob, len(blob))
+    domain.check(domain.lib.ssg_load_state(domain.handle, buf, len(blob), ctypes.byref(state._c)))
+
+
+def step_until(fields: Fields, domain: Domain, params: "TimeParams", state: "TimeState", stop_time: float) -> int:
+    """``step_until!`` (src/timestep.jl:408-449): steps of min(dt, time left)
+    until ``stop_time``; the clock ends exactly at ``stop_time``.  Returns
+    the number of steps taken."""
+    if not stop_time > state.t:
+        raise ValueError(f"stop_time ({stop_time}) must be greater than current time ({state.t})")
+    n = 0
+    while state.t < stop_time:
+        dt_use = min(params.dt, stop_time - state.t)
+        p = TimeParams(dt_use, scheme=params.scheme, filter_freq=params.filter_freq,
+                       filter_strength=params.filter_strength, filter_order=params.filter_order,
+                       filter_cutoff=params.filter_cutoff, ab2_reference_aliasing=params.ab2_reference_aliasing,
+                       check_finite=params.check_finite)
+        timestep(fields, domain, p, state, 1)
+        n += 1
+    state.t = stop_time
+    return n
 
 
 def ma_adjoint(domain: Domain, phibar, *, eps: float = 0.1, tol: float = 1e-10, maxiter: int = 50,

@@ -1,0 +1,113 @@
+"""Setup, drivers and restart around the step (the callers either side of
+timestep!): set_b! (src/setICs.jl:21-88), compute_cfl_number /
+adaptive_timestep (src/timestep.jl:91-147), step_until!
+(src/timestep.jl:408-449) and an exact restart of the step state."""
+import numpy as np
+import pytest
+
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def test_set_b_matches_oracle_pipeline(gpu):
+    """b <- irfft(mask · rfft(b)) with the k = 0 mode removed, then the
+    Monge-Ampère solve for φₛ and the velocities."""
+    n, nz = 64, 8
+    rng = np.random.default_rng(12)
+    b = rng.standard_normal((n, n)) + 3.0          # not band-limited, nonzero mean
+    o = Oracle(n, n, nz)
+    s = o.rfft2(b)
+    s[0, 0] = 0.0
+    expect = o.irfft2(o.dealias2(s))
+    d = gpu.make_domain(n, n, nz)
+    f = gpu.allocate_fields(d)
+    assert gpu.set_b(f, d, b, eps=None) is None
+    assert np.abs(f.bs - expect).max() < 1e-12 * np.abs(expect).max()
+    assert abs(f.bs.mean()) < 1e-14
+    diag = gpu.set_b(f, d, b, eps=0.1, tol=1e-12, maxiter=60)
+    assert diag["converged"]
+    phis = f.phis
+    # the same as an explicit solve on the set buoyancy
+    d2 = gpu.make_domain(n, n, nz)
+    f2 = gpu.allocate_fields(d2)
+    f2.bs = expect
+    gpu.solve_monge_ampere(f2, d2, eps=0.1, tol=1e-12, maxiter=60)
+    assert np.abs(phis - f2.phis).max() < 1e-10 * np.abs(phis).max()
+    assert np.abs(f.u - f2.u).max() < 1e-10 * np.abs(f.u).max()
+    d.close()
+    d2.close()
+
+
+def test_cfl_and_adaptive_dt(gpu):
+    from ssgamd.ics import spectral_b0, streamfunction_from_b
+    n = 128
+    d = gpu.make_domain(n, n, 1)
+    f = gpu.allocate_fields(d)
+    f.phis = streamfunction_from_b(spectral_b0(n, n, seed=2))
+    dt = 0.05
+    cfl = gpu.compute_cfl_number(f, d, dt)
+    dx = 2 * np.pi / n
+    expect = max(np.abs(f.u).max() * dt / dx, np.abs(f.v).max() * dt / dx)
+    assert abs(cfl - expect) < 1e-14 * expect
+    # adaptive_timestep: dt scaled to the CFL safety bound
+    p = gpu.TimeParams(dt, scheme=gpu.RK3, adaptive_dt=True, cfl_safety=0.5 * cfl, min_dt=1e-8, max_dt=1.0)
+    st = gpu.TimeState(0.0, f)
+    f.bs = spectral_b0(n, n, seed=3)
+    used = gpu.timestep(f, d, p, st, 4)
+    assert abs(used - dt * 0.5) < 1e-15 and abs(st.t - 4 * used) < 1e-14 and abs(st.cfl_max - cfl) < 1e-14
+    d.close()
+
+
+def test_step_until_lands_on_stop_time(gpu):
+    from ssgamd.ics import spectral_b0, streamfunction_from_b
+    n = 64
+    d = gpu.make_domain(n, n, 1)
+    f = gpu.allocate_fields(d)
+    b0 = spectral_b0(n, n, seed=4)
+    f.phis = streamfunction_from_b(b0)
+    f.bs = b0
+    st = gpu.TimeState(0.0, f)
+    p = gpu.TimeParams(1e-3, scheme=gpu.RK3, filter_freq=4, filter_strength=36.0, filter_order=8, filter_cutoff=0.65)
+    nsteps = gpu.step_until(f, d, p, st, 0.0105)
+    assert nsteps == 11 and st.step == 11 and st.t == 0.0105
+    # the oracle: ten steps of dt and one of 0.5 dt
+    o = Oracle(n, n)
+    bo = b0.copy()
+    for k in range(11):
+        o.timestep(bo, f.phis, 1, 1e-3 if k < 10 else 0.0105 - o.t.value, 4, 36.0, 8, 0.65, 0)
+    assert np.abs(f.bs - bo).max() < 1e-11 * np.abs(bo).max()
+    d.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_restart_is_bitwise(gpu, prec, scheme):
+    """Saved after 7 steps and loaded into a fresh context, 5 more steps give
+    bitwise the uninterrupted run (the AB2 history travels with the state)."""
+    from ssgamd.ics import spectral_b0, streamfunction_from_b
+    n = 128
+    b0 = spectral_b0(n, n, seed=5)
+    psi = streamfunction_from_b(b0)
+    p = gpu.TimeParams(1e-3, scheme=scheme, filter_freq=3, filter_strength=36.0, filter_order=8, filter_cutoff=0.65)
+    d = gpu.make_domain(n, n, 1, precision=prec)
+    f = gpu.allocate_fields(d)
+    f.phis = psi
+    f.bs = b0
+    st = gpu.TimeState(0.0, f)
+    gpu.timestep(f, d, p, st, 7)
+    blob = gpu.save_state(d, st)
+    gpu.timestep(f, d, p, st, 5)
+    ref = f.bs
+    d2 = gpu.make_domain(n, n, 1, precision=prec)
+    f2 = gpu.allocate_fields(d2)
+    st2 = gpu.TimeState(0.0, f2)
+    gpu.load_state(d2, blob, st2)
+    assert st2.step == 7
+    gpu.timestep(f2, d2, p, st2, 5)
+    assert np.array_equal(f2.bs, ref) and st2.step == st.step and st2.t == st.t
+    with pytest.raises(gpu.SSGError):     # a blob of another grid is rejected
+        d3 = gpu.make_domain(64, 64, 1, precision=prec)
+        gpu.load_state(d3, blob, gpu.TimeState())
+    d.close()
+    d2.close()
