@@ -573,6 +573,31 @@ def step_until(fields: Fields, domain: Domain, params: "TimeParams", state: "Tim
     return n
 
 
+class DiagnosticTimeSeries:
+    """``DiagnosticTimeSeries`` (src/timestep.jl:289-298): times, kinetic
+    energy, enstrophy, total buoyancy, max divergence and max CFL, one entry
+    per :func:`update_diagnostics` call.  As in the reference, enstrophy and
+    max divergence are recorded as 0 placeholders."""
+
+    def __init__(self):
+        self.times, self.kinetic_energy, self.enstrophy = [], [], []
+        self.total_buoyancy, self.max_divergence, self.max_cfl = [], [], []
+
+
+def update_diagnostics(diag: DiagnosticTimeSeries, fields: Fields, domain: Domain, state: "TimeState"):
+    """``update_diagnostics!`` (src/timestep.jl:303-330) from one
+    ``ssg_diagnostics`` reduction on the device: the surface kinetic energy
+    (the ``compute_surface_kinetic_energy`` formula), Σb and ``state.cfl_max``."""
+    d = diagnostics(domain)
+    diag.times.append(state.t)
+    diag.kinetic_energy.append(d["ke_surface"])
+    diag.enstrophy.append(0.0)
+    diag.total_buoyancy.append(d["sum_b"])
+    diag.max_divergence.append(0.0)
+    diag.max_cfl.append(getattr(state, "cfl_max", 0.0))
+    return diag
+
+
 def ma_adjoint(domain: Domain, phibar, *, eps: float = 0.1, tol: float = 1e-10, maxiter: int = 50,
                variant: int = MA_BODY) -> tuple:
     """Vector-Jacobian product of the last Monge-Ampère solve:

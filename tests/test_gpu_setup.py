@@ -111,3 +111,24 @@ def test_restart_is_bitwise(gpu, prec, scheme):
         gpu.load_state(d3, blob, gpu.TimeState())
     d.close()
     d2.close()
+
+
+def test_diagnostic_time_series(gpu):
+    from ssgamd.ics import spectral_b0, streamfunction_from_b
+    n = 64
+    d = gpu.make_domain(n, n, 1)
+    f = gpu.allocate_fields(d)
+    b0 = spectral_b0(n, n, seed=6)
+    f.phis = streamfunction_from_b(b0)
+    f.bs = b0
+    st = gpu.TimeState(0.0, f)
+    ts = gpu.DiagnosticTimeSeries()
+    p = gpu.TimeParams(1e-3, scheme=gpu.RK3, filter_freq=0)
+    for _ in range(3):
+        gpu.timestep(f, d, p, st, 5)
+        gpu.update_diagnostics(ts, f, d, st)
+    assert len(ts.times) == 3 and abs(ts.times[-1] - 0.015) < 1e-15
+    # frozen φₛ: the kinetic energy is constant and Σb is conserved
+    assert max(ts.kinetic_energy) - min(ts.kinetic_energy) < 1e-12 * ts.kinetic_energy[0]
+    assert max(abs(x - b0.sum()) for x in ts.total_buoyancy) < 1e-10
+    d.close()
