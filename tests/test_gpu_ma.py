@@ -248,3 +248,57 @@ def test_coupled_rk3_mixed_against_fp64(gpu):
     print(f"coupled mixed vs fp64 after {steps} steps: b {eb:.2e} phis {ep:.2e} ke {ek:.2e}", flush=True)
     assert eb < 1e-5 and ep < 1e-5 and ek < 1e-5
     assert eb > 0   # the FP32 path really ran
+
+
+@pytest.mark.parametrize("nx,ny,nz,variant,k0", [(16, 8, 4, 1, 1.0), (8, 16, 4, 0, 1.5), (8, 16, 4, 1, 1.5),
+                                                  (8, 32, 3, 0, 1.5)])
+def test_rectangular_grids_newton(gpu, nx, ny, nz, variant, k0):
+    """Non-square horizontal grids: the row pass (along x) and the column
+    pass (along y) have different lengths; nz = 3 is the smallest column
+    (one interior level between the two boundary rows)."""
+    eps = 0.1
+    from ssgamd.ics import spectral_b0
+    b = spectral_b0(nx, ny, seed=8, k0=k0) * (1.0 if variant == 0 else 0.3)
+    d = gpu.make_domain(nx, ny, nz)
+    f = gpu.allocate_fields(d)
+    f.bs = b
+    diag = gpu.solve_monge_ampere(f, d, eps=eps, tol=1e-12, maxiter=80, variant=variant)
+    assert diag["converged"], diag
+    phi = gpu.get_phi3d(d)
+    o = Oracle(nx, ny, nz)
+    b3d = np.stack([b] * nz) if variant == 0 else np.zeros((nz, ny, nx))
+    g = None if variant == 0 else b
+    ref, _ = ma_newton(o, b3d, g, eps)
+    assert np.abs(demean(phi) - demean(ref)).max() < 1e-9 * np.abs(ref).max()
+    r, _ = o.ma_residual(phi, b3d, g, eps)
+    assert np.abs(r).max() < 1e-9 * max(1.0, np.abs(b).max())
+    d.close()
+
+
+def test_coupled_ab2_against_oracle(gpu):
+    """Coupled AB2 (forward Euler first step): the Monge-Ampère problem
+    re-solved before every tendency, against the oracle tendency fed by
+    dense-Newton solves."""
+    n, nz, eps, dt = 8, 4, 0.1, 1e-2
+    b0 = smooth_b(n, 9, 1.2)
+    d = gpu.make_domain(n, n, nz)
+    f = gpu.allocate_fields(d)
+    f.bs = b0
+    gpu.set_coupling(d, eps=eps, tol=1e-13, maxiter=80, variant=gpu.MA_BODY)
+    st = gpu.TimeState(0.0, f)
+    gpu.timestep(f, d, gpu.TimeParams(dt, scheme=gpu.AB2_LowStorage, filter_freq=0), st, 3)
+    g = f.bs
+    o = Oracle(n, n, nz)
+
+    def psi_of(b):
+        phi, _ = ma_newton(o, np.stack([b] * nz), None, eps)
+        return phi[-1] - phi[-1].mean()
+
+    b = b0.copy()
+    fold = None
+    for k in range(3):
+        t = o.tendency(psi_of(b), b)
+        b = b + dt * t if k == 0 else b + dt * (1.5 * t - 0.5 * fold)
+        fold = t
+    assert np.abs(g - b).max() < 1e-9 * np.abs(b).max()
+    d.close()
