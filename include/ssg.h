@@ -175,11 +175,13 @@ int ssg_get_phi3d(ssg_ctx* ctx, double* host, int64_t n);
    Monge-Ampère problem for the stage buoyancy before every tendency
    evaluation (φₛ then follows b); NULL restores the reference's frozen φₛ. */
 int ssg_set_coupling(ssg_ctx* ctx, const ssg_ma_params* params);
-/* Discrete adjoint (vector-Jacobian product) of `nsteps` RK3 timestep!
-   calls with frozen φₛ that ended at step number `step_end`: lam_out =
-   (∂b(step_end)/∂b(step_end - nsteps))ᵀ lam_in, with the same filter
-   cadence.  The step is linear in b, so no forward trajectory is needed.
-   lam_in, lam_out: host, nx*ny FP64 (may alias).  Arithmetic is FP64. */
+/* Discrete adjoint (vector-Jacobian product) of `nsteps` timestep! calls
+   (RK3, src/timestep.jl:195-242, or AB2, :152-190, including the aliased
+   tendency of its call site) with frozen φₛ that ended at step number
+   `step_end`: lam_out = (∂b(step_end)/∂b(step_end - nsteps))ᵀ lam_in, with
+   the same filter cadence; for AB2 the previous tendency at the window's
+   start is held fixed.  The step is linear in b, so no forward trajectory
+   is needed.  lam_in, lam_out: host, nx*ny FP64 (may alias).  FP64. */
 int ssg_adjoint(ssg_ctx* ctx, const ssg_time_params* params, int64_t step_end, int64_t nsteps, const double* lam_in,
                 double* lam_out);
 

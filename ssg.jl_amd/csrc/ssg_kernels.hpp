@@ -40,7 +40,10 @@ enum {
     EPI_ADJ1 = 10,   // lam = acc + g
     // coupled-step adjoint (MODE_PLANES): out_r = u A + v B (the frozen-w
     // transpose), w_out = (b A, b B) (the adjoint of the velocities)
-    EPI_ADJW = 11
+    EPI_ADJW = 11,
+    // transposed tendency alone (MODE_PLANES): out_r = u A + v B, or 2 v B for
+    // the AB2 call site's aliased tendency
+    EPI_ADJG = 12
 };
 enum {
     OP_TEND = 0,     // -M (Dx Fx + Dy Fy)
@@ -568,6 +571,7 @@ struct RowsInvArgs {
     double* sbuf;          // adjoint: next stage's input
     double c1, c2, c3;     // stage coefficients
     double gscale;         // adjoint: scale of T^T s
+    int adj_quirk;         // EPI_ADJG: transpose of the aliased AB2 tendency
     int first;             // AB2: forward Euler on step 0
     // next-stage flux emission
     int emit;
@@ -698,6 +702,11 @@ __device__ __forceinline__ void rows_inv_body(const RowsInvArgs<R>& a, const int
             } else {
                 a.lam[bA + x] = a.acc[bA + x] + g;
             }
+        } else if constexpr (EPI == EPI_ADJG) {
+            const C uv = a.w[bA + x];
+            const double g = a.adj_quirk ? 2.0 * (double)uv.y * (double)tB
+                                         : (double)uv.x * (double)tA + (double)uv.y * (double)tB;
+            a.out_r[bA + x] = (R)(a.gscale * g);
         } else if constexpr (EPI == EPI_ADJW) {
             const C uv = a.w[bA + x];
             const double bb = a.b[bA + x];

@@ -451,6 +451,63 @@ int adjoint_steps(ssg_ctx* c, const ssg_time_params* p, int64_t step_end, int64_
     return SSG_OK;
 }
 
+int lincomb(ssg_ctx* c, double* out, double ca, const double* x, double cb, const double* y) {
+    const long n = (long)c->nx * c->ny;
+    hipLaunchKernelGGL(k_lincomb, dim3(grid1d(n)), dim3(256), 0, c->stream, out, ca, x, cb, y, n);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    return SSG_OK;
+}
+
+// AB2 (frozen φₛ).  State (b, f_old); step n: b' = b + dt(1.5 T b - 0.5 f_old),
+// f_old' = T b, with forward Euler b' = b + dt T b on the step from step 0
+// (src/timestep.jl:152-190).  Transpose with adjoint state (λ_b, λ_f):
+//   s = c λ_b + λ_f  (c = 1.5 dt, or dt on the Euler step)
+//   λ_f <- -0.5 dt λ_b (0 on the Euler step),  λ_b <- λ_b + Tᵀ s
+// The aliased call site's tendency T_Q b = -2 M Dy(v b) transposes to
+// 2 v Dy(M s).  λ_f starts at 0: the objective reads b only.
+int adj_tt(ssg_ctx* c, const double* sin, double* out, int quirk) {
+    PathBuffers<double>& D = c->d;
+    int rc = launch_rows_fwd<double, double, SRC_PAIR>(c, sin, nullptr, D.Tp, nullptr, c->ncols_tend);
+    if (rc) return rc;
+    ColsArgs<double> a{};
+    a.inA = D.Tp;
+    a.outA = D.Fx;
+    a.outB = D.Fy;
+    a.scale = 1.0 / ((double)c->nx * c->ny);
+    if ((rc = launch_cols_args<double, OP_ADJ>(c, a, c->ncols_tend, 1))) return rc;
+    RowsInvArgs<double> r{};
+    r.P0 = D.Fx;
+    r.P1 = D.Fy;
+    r.ncols_in = c->ncols_tend;
+    r.w = D.w;
+    r.out_r = out;
+    r.gscale = 1.0;
+    r.adj_quirk = quirk;
+    return launch_rows_inv<double, MODE_PLANES, EPI_ADJG>(c, r);
+}
+
+int adjoint_steps_ab2(ssg_ctx* c, const ssg_time_params* p, int64_t step_end, int64_t nsteps) {
+    const double dt = p->dt;
+    const int quirk = p->ab2_reference_aliasing ? 1 : 0;
+    double* lf = c->adj_acc;
+    double* sb = c->adj_s;
+    double* g = c->dscratch;
+    HIP_OK(c, hipMemsetAsync(lf, 0, (size_t)c->nx * c->ny * sizeof(double), c->stream));
+    int rc;
+    for (int64_t n = step_end; n > step_end - nsteps; --n) {
+        if (p->filter_freq > 0 && (n % p->filter_freq) == 0)
+            if ((rc = filter_b<double>(c, p->filter_strength, p->filter_order, p->filter_cutoff, 0, c->adj_lam)))
+                return rc;
+        const bool euler = (n - 1) == 0;
+        if ((rc = lincomb(c, sb, euler ? dt : 1.5 * dt, c->adj_lam, 1.0, lf))) return rc;
+        if ((rc = adj_tt(c, sb, g, quirk))) return rc;
+        if ((rc = lincomb(c, lf, euler ? 0.0 : -0.5 * dt, c->adj_lam, 0.0, nullptr))) return rc;
+        if ((rc = lincomb(c, c->adj_lam, 1.0, c->adj_lam, 1.0, g))) return rc;
+    }
+    return SSG_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Discrete adjoint of the coupled step (Monge-Ampère solve at every stage),
 // FP64.  Stage s maps b_s -> Φ_s = MA(b_s) -> w_s = (-Dy φ_s, Dx φ_s) with
@@ -509,13 +566,6 @@ int adjc_alloc(ssg_ctx* c) {
     return SSG_OK;
 }
 
-int lincomb(ssg_ctx* c, double* out, double ca, const double* x, double cb, const double* y) {
-    const long n = (long)c->nx * c->ny;
-    hipLaunchKernelGGL(k_lincomb, dim3(grid1d(n)), dim3(256), 0, c->stream, out, ca, x, cb, y, n);
-    c->launches++;
-    HIP_OK(c, hipGetLastError());
-    return SSG_OK;
-}
 
 // inverse 2D transform of a [kx][ky] spectrum (unnormalized forward
 // convention) into a real field
@@ -936,8 +986,10 @@ int ssg_adjoint(ssg_ctx* c, const ssg_time_params* p, int64_t step_end, int64_t 
                 double* lam_out) {
     if (!c) return SSG_ERR_ARG;
     if (!p || !lam_in || !lam_out) return set_err(c, SSG_ERR_ARG, "null pointer");
-    if (p->scheme != SSG_RK3) return set_err(c, SSG_ERR_UNSUPPORTED, "the adjoint covers the RK3 scheme");
-    if (c->coupled) return set_err(c, SSG_ERR_UNSUPPORTED, "the adjoint covers the frozen-φₛ step");
+    if (p->scheme != SSG_RK3 && p->scheme != SSG_AB2_LOWSTORAGE)
+        return set_err(c, SSG_ERR_UNSUPPORTED, "the adjoint covers the RK3 and AB2 schemes");
+    if (c->coupled)
+        return set_err(c, SSG_ERR_UNSUPPORTED, "the coupled step's adjoint is ssg_adjoint_coupled");
     if (nsteps < 0 || step_end < nsteps) return set_err(c, SSG_ERR_ARG, "need 0 <= nsteps <= step_end");
     if (!(p->dt > 0)) return set_err(c, SSG_ERR_ARG, "dt must be > 0");
     HIP_OK(c, hipSetDevice(c->device));
@@ -945,7 +997,9 @@ int ssg_adjoint(ssg_ctx* c, const ssg_time_params* p, int64_t step_end, int64_t 
     if (rc) return rc;
     const size_t bytes = (size_t)c->nx * c->ny * sizeof(double);
     HIP_OK(c, hipMemcpyAsync(c->adj_lam, lam_in, bytes, hipMemcpyHostToDevice, c->stream));
-    if ((rc = adjoint_steps(c, p, step_end, nsteps))) return rc;
+    if ((rc = p->scheme == SSG_RK3 ? adjoint_steps(c, p, step_end, nsteps)
+                                   : adjoint_steps_ab2(c, p, step_end, nsteps)))
+        return rc;
     HIP_OK(c, hipMemcpyAsync(lam_out, c->adj_lam, bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     return SSG_OK;

@@ -88,9 +88,32 @@ def test_gradient_against_finite_differences(gpu):
         assert abs(fd - an) <= 1e-6 * abs(an)
 
 
+@pytest.mark.parametrize("aliasing", [False, True])
+@pytest.mark.parametrize("start", [0, 3])
+def test_ab2_dot_product_against_oracle_step(gpu, aliasing, start):
+    """AB2 windows from step 0 (forward Euler first) and from step 3 (the
+    previous tendency held fixed: the oracle starts from a zero history),
+    plain and with the aliased tendency of the reference's call site."""
+    n, nsteps, dt, ff = 64, 8, 2e-2, 4
+    d, f, b0, psi = setup(gpu, n)
+    params = gpu.TimeParams(dt, scheme=gpu.AB2_LowStorage, filter_freq=ff, filter_strength=36.0, filter_order=8,
+                            filter_cutoff=0.65, ab2_reference_aliasing=aliasing)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((n, n))
+    lam = rng.standard_normal((n, n))
+    o = Oracle(n, n)
+    o.step.value = start
+    y = x.copy()
+    for _ in range(nsteps):
+        o.timestep(y, psi, 0, dt, ff, 36.0, 8, 0.65, int(aliasing))
+    adj = gpu.timestep_adjoint(d, params, start + nsteps, nsteps, lam)
+    lhs, rhs = np.sum(y * lam), np.sum(x * adj)
+    assert abs(lhs - rhs) < 1e-11 * np.sqrt(np.sum(y ** 2) * np.sum(lam ** 2))
+
+
 def test_adjoint_rejects_unsupported(gpu):
     d, f, b0, psi = setup(gpu, 32)
     with pytest.raises(gpu.SSGError):
-        gpu.timestep_adjoint(d, gpu.TimeParams(1e-3, scheme=gpu.AB2_LowStorage), 4, 4, np.zeros((32, 32)))
+        gpu.timestep_adjoint(d, gpu.TimeParams(1e-3, scheme=gpu.RK3_LowStorage), 4, 4, np.zeros((32, 32)))
     with pytest.raises(gpu.SSGError):
         gpu.timestep_adjoint(d, gpu.TimeParams(1e-3, scheme=gpu.RK3), 2, 4, np.zeros((32, 32)))
