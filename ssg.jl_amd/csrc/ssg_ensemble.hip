@@ -21,7 +21,9 @@ struct ssg_ensemble {
     double tune_us[3] = {-1.0, -1.0, -1.0};
     // device argument arrays: [0] cols TEND, rows RK1, RK2, RK3 emit, RK3 no emit
     void* d_cols = nullptr;
-    void* d_rows[4] = {nullptr, nullptr, nullptr, nullptr};
+    // rows argument sets: [0] RK1, [1] RK2, [2] RK3 emit, [3] RK3 no emit,
+    // [4] AB2 emit, [5] AB2 no emit, [6] AB2 forward Euler emit, [7] no emit
+    void* d_rows[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int2* d_wcols = nullptr;
     int2* d_wrows = nullptr;
     int ncb = 0, nrb = 0;
@@ -73,7 +75,7 @@ template <typename R>
 int ens_build(ssg_ensemble* e, int E, const double* dt) {
     ens_free_tables(e);
     std::vector<ColsArgs<R>> ca(e->n);
-    std::vector<RowsInvArgs<R>> ra[4];
+    std::vector<RowsInvArgs<R>> ra[8];
     for (auto& v : ra) v.resize(e->n);
     std::vector<int2> wc, wr;
     size_t lc = 0, lr = 0;
@@ -97,8 +99,9 @@ int ens_build(ssg_ensemble* e, int E, const double* dt) {
         const int nbc = (c->ncols_tend + Gc - 1) / Gc;
         for (int j = 0; j < nbc; ++j) wc.push_back(int2{i, j});
         lc = std::max(lc, (size_t)Gc * 2 * lds_line_elems(c->ny) * sizeof(cpx<R>));
-        const double coef[4] = {dt[i] / 3.0, 2.0 * dt[i] / 3.0, dt[i] / 4.0, dt[i] / 4.0};
-        for (int k = 0; k < 4; ++k) {
+        const double coef[8] = {dt[i] / 3.0, 2.0 * dt[i] / 3.0, dt[i] / 4.0, dt[i] / 4.0,
+                                dt[i],       dt[i],             dt[i],       dt[i]};
+        for (int k = 0; k < 8; ++k) {
             RowsInvArgs<R> r{};
             r.P0 = B.Tp;
             r.ncols_in = c->ncols_tend;
@@ -106,7 +109,8 @@ int ens_build(ssg_ensemble* e, int E, const double* dt) {
             r.k1 = B.k1;
             r.fold = B.fold;
             r.c1 = coef[k];
-            r.emit = k == 3 ? 0 : 1;
+            r.emit = (k == 3 || k == 5 || k == 7) ? 0 : 1;
+            r.first = k >= 6 ? 1 : 0;
             r.w = B.w;
             r.Fx = B.Fx;
             r.Fy = B.Fy;
@@ -122,7 +126,7 @@ int ens_build(ssg_ensemble* e, int E, const double* dt) {
     }
     ENS_OK(e, hipMalloc(&e->d_cols, ca.size() * sizeof(ColsArgs<R>)));
     ENS_OK(e, hipMemcpy(e->d_cols, ca.data(), ca.size() * sizeof(ColsArgs<R>), hipMemcpyHostToDevice));
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 8; ++k) {
         ENS_OK(e, hipMalloc(&e->d_rows[k], ra[k].size() * sizeof(RowsInvArgs<R>)));
         ENS_OK(e, hipMemcpy(e->d_rows[k], ra[k].data(), ra[k].size() * sizeof(RowsInvArgs<R>), hipMemcpyHostToDevice));
     }
@@ -151,6 +155,8 @@ int ens_launch_stage(ssg_ensemble* e, int stage_k) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void*)k_rows_inv_batch<R, E, EPI_RK3>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)k_rows_inv_batch<R, E, EPI_AB2>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
     hipLaunchKernelGGL((k_cols_batch<R, E, OP_TEND>), dim3(e->ncb), dim3(256), e->lds_cols, e->stream,
@@ -162,6 +168,9 @@ int ens_launch_stage(ssg_ensemble* e, int stage_k) {
                            e->d_wrows);
     else if (stage_k == 1)
         hipLaunchKernelGGL((k_rows_inv_batch<R, E, EPI_RK2>), dim3(e->nrb), dim3(256), e->lds_rows, e->stream, ra,
+                           e->d_wrows);
+    else if (stage_k >= 4)
+        hipLaunchKernelGGL((k_rows_inv_batch<R, E, EPI_AB2>), dim3(e->nrb), dim3(256), e->lds_rows, e->stream, ra,
                            e->d_wrows);
     else
         hipLaunchKernelGGL((k_rows_inv_batch<R, E, EPI_RK3>), dim3(e->nrb), dim3(256), e->lds_rows, e->stream, ra,
@@ -199,10 +208,17 @@ template <typename R>
 int ens_issue_step(ssg_ensemble* e, const ssg_time_params* p, int64_t step_before, bool last) {
     const int64_t step_after = step_before + 1;
     const bool filt = p->filter_freq > 0 && (step_after % p->filter_freq) == 0;
+    const bool emit = !filt && !last;
     int rc;
-    if ((rc = ens_stage<R>(e, 0))) return rc;
-    if ((rc = ens_stage<R>(e, 1))) return rc;
-    if ((rc = ens_stage<R>(e, (!filt && !last) ? 2 : 3))) return rc;
+    if (p->scheme == SSG_AB2_LOWSTORAGE) {
+        // one tendency per step; forward Euler on the step from step 0
+        // (src/timestep.jl:152-190)
+        if ((rc = ens_stage<R>(e, 4 + (step_before == 0 ? 2 : 0) + (emit ? 0 : 1)))) return rc;
+    } else {
+        if ((rc = ens_stage<R>(e, 0))) return rc;
+        if ((rc = ens_stage<R>(e, 1))) return rc;
+        if ((rc = ens_stage<R>(e, emit ? 2 : 3))) return rc;
+    }
     if (filt) {
         for (ssg_ctx* c : e->m) {
             if ((rc = filter_b<R>(c, p->filter_strength, p->filter_order, p->filter_cutoff, last ? 0 : 1)))
@@ -285,7 +301,9 @@ int ens_run(ssg_ensemble* e, const ssg_time_params* p, const double* dt, ssg_tim
     while (it < nsteps) {
         const int64_t remaining = nsteps - it;
         int adv = 1;
-        if (graphs && (p->filter_freq <= 0 || step % F == 0) && remaining > F) {
+        // the forward-Euler first AB2 step is never part of a replayed cycle
+        const bool euler_ahead = p->scheme == SSG_AB2_LOWSTORAGE && step == 0;
+        if (graphs && !euler_ahead && (p->filter_freq <= 0 || step % F == 0) && remaining > F) {
             if (!e->graph || e->graph_len != F || !same_params(e->graph_p, *p)) {
                 if (e->graph) {
                     (void)hipGraphExecDestroy(e->graph);
@@ -318,6 +336,8 @@ int ens_run(ssg_ensemble* e, const ssg_time_params* p, const double* dt, ssg_tim
                 st[k].step += 1;
             }
         for (int k = 0; k < e->n; ++k) st[k].dt_actual = dt[k];
+        if (p->scheme == SSG_AB2_LOWSTORAGE)
+            for (ssg_ctx* c : e->m) c->ab2_have_history = true;
         step += adv;
         it += adv;
     }
@@ -383,12 +403,18 @@ int ssg_ensemble_timestep(ssg_ensemble* e, const ssg_time_params* p, const doubl
                           int64_t nsteps) {
     if (!e) return SSG_ERR_ARG;
     if (!p || !dt || !st) return ens_err(e, SSG_ERR_ARG, "null pointer");
-    if (p->scheme != SSG_RK3) return ens_err(e, SSG_ERR_UNSUPPORTED, "the ensemble step covers RK3");
+    if (p->scheme != SSG_RK3 && p->scheme != SSG_AB2_LOWSTORAGE)
+        return ens_err(e, SSG_ERR_UNSUPPORTED, "the ensemble step covers RK3 and AB2");
+    if (p->scheme == SSG_AB2_LOWSTORAGE && p->ab2_reference_aliasing)
+        return ens_err(e, SSG_ERR_UNSUPPORTED, "the ensemble's AB2 uses the corrected tendency");
     if (nsteps < 0) return ens_err(e, SSG_ERR_ARG, "nsteps < 0");
     for (int i = 0; i < e->n; ++i) {
         if (!(dt[i] > 0) || !std::isfinite(dt[i])) return ens_err(e, SSG_ERR_ARG, "dt[%d] must be finite and > 0", i);
         if (st[i].step != st[0].step) return ens_err(e, SSG_ERR_ARG, "members must be at the same step number");
         if (e->m[i]->coupled) return ens_err(e, SSG_ERR_UNSUPPORTED, "member %d is in coupled mode", i);
+        if (p->scheme == SSG_AB2_LOWSTORAGE && st[i].step > 0 && !e->m[i]->ab2_have_history)
+            return ens_err(e, SSG_ERR_ARG, "AB2 step %lld of member %d needs the previous tendency; none is held",
+                           (long long)st[i].step, i);
     }
     if (p->filter_freq > 0 &&
         (!(p->filter_strength >= 0) || p->filter_order < 1 || !(p->filter_cutoff > 0 && p->filter_cutoff <= 1)))

@@ -11,6 +11,10 @@ int ma_alloc(ssg_ctx* c) {
     if (c->ma_ready) return SSG_OK;
     if (c->nz < 3 || c->nz > 128)
         return set_err(c, SSG_ERR_UNSUPPORTED, "the Monge-Ampere solve needs 3 <= nz <= 128 (nz = %d)", c->nz);
+    // the y-pass transforms three lines in lockstep: 3 FP64 line buffers of
+    // length ny must fit one CU's 160 KiB of LDS
+    if (3 * (size_t)lds_line_elems(c->ny) * sizeof(cpx<double>) > 160 * 1024)
+        return set_err(c, SSG_ERR_UNSUPPORTED, "the Monge-Ampere solve supports ny <= 2048 (ny = %d)", c->ny);
     const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
     HIP_OK(c, hipMalloc(&c->ma_phi, c3 * sizeof(cpx<double>)));
     HIP_OK(c, hipMalloc(&c->ma_dprev, c3 * sizeof(cpx<double>)));

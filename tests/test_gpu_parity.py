@@ -131,6 +131,15 @@ def test_rk3_steps_match_oracle(gpu, nx, ny):
     assert rel(g, o) < TOL64
 
 
+@pytest.mark.parametrize("nx,ny", [(4096, 8), (8, 4096), (2048, 16)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_longest_lines_match_oracle(gpu, nx, ny, prec):
+    """Lines of 4096 points (the largest supported axis) along x and along y."""
+    print(f"long lines {nx}x{ny} prec={prec}", flush=True)
+    g, o = run_pair(gpu, nx, ny, gpu.RK3, 6, 1e-4, prec=prec, filter_freq=3)
+    assert rel(g, o) < (TOL64 if prec == 0 else TOL32)
+
+
 @pytest.mark.parametrize("aliasing", [False, True])
 def test_ab2_steps_match_oracle(gpu, aliasing):
     g, o = run_pair(gpu, 64, 64, gpu.AB2_LowStorage, 23, 2.5e-4, aliasing=aliasing, chunks=(1, 5, None))
