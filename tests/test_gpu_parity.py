@@ -255,3 +255,42 @@ def test_graph_replay_matches_eager(gpu, monkeypatch):
         d.close()
     for a, b in zip(runs["1"], runs["0"]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("scheme", [1, 0])
+def test_precision_configuration_against_oracle(gpu, scheme):
+    """The precision configuration at full size: 1024², 1000 steps of RK3
+    (the metric's scheme) or AB2 (the configuration's), dt = 2.5e-4, filter
+    α = 36, p = 8, c = 0.65 every 10 steps.  Both GPU forms against the
+    FP64 oracle on the same inputs: surface buoyancy rel-L2 (≤ 1e-5 for
+    FP32-mixed, the metric's bar) and the buoyancy variance ½Σb²."""
+    from oracle import set_num_threads
+    from ssgamd.ics import spectral_b0, streamfunction_from_b
+    import os
+    n, steps, dt = 1024, 1000, 2.5e-4
+    b0 = spectral_b0(n, n, seed=0)
+    psi = streamfunction_from_b(b0)
+    params = gpu.TimeParams(dt, scheme=scheme, filter_freq=10, filter_strength=36.0, filter_order=8,
+                            filter_cutoff=0.65)
+    outs = {}
+    for prec in (0, 1):
+        d = gpu.make_domain(n, n, 32, precision=prec)
+        f = gpu.allocate_fields(d)
+        f.phis = psi
+        f.bs = b0
+        st = gpu.TimeState(0.0, f)
+        gpu.timestep(f, d, params, st, steps)
+        outs[prec] = f.bs
+        d.close()
+    set_num_threads(min(16, os.cpu_count() or 1))
+    o = Oracle(n, n)
+    bo = b0.copy()
+    for _ in range(steps):
+        assert o.timestep(bo, psi, scheme, dt, 10, 36.0, 8, 0.65, 0) == 0
+    e64 = np.linalg.norm(outs[0] - bo) / np.linalg.norm(bo)
+    e32 = np.linalg.norm(outs[1] - bo) / np.linalg.norm(bo)
+    v32 = abs(np.sum(outs[1] ** 2) - np.sum(bo ** 2)) / np.sum(bo ** 2)
+    print(f"scheme {scheme}, {steps} steps at {n}^2: FP64 rel-L2 {e64:.2e}, FP32-mixed rel-L2 {e32:.2e}, "
+          f"variance {v32:.2e}", flush=True)
+    assert e64 < 1e-11
+    assert e32 < TOL32 and v32 < TOL32
