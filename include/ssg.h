@@ -192,13 +192,14 @@ int ssg_adjoint(ssg_ctx* ctx, const ssg_time_params* params, int64_t step_end, i
    phibar, bbar: host, nx*ny FP64.  diag (optional) reports the transposed
    iteration; SSG_ERR_NOT_CONVERGED if it misses params->tol. */
 int ssg_ma_adjoint(ssg_ctx* ctx, const ssg_ma_params* params, const double* phibar, double* bbar, ssg_ma_diag* diag);
-/* Discrete adjoint of `nsteps` coupled RK3 steps (Monge-Ampère solve at
-   every stage, params `ma`), starting from the context's current b at step
+/* Discrete adjoint of `nsteps` coupled steps (RK3, or AB2 with the
+   corrected tendency; Monge-Ampère solve before every tendency, params
+   `ma`), starting from the context's current b (and AB2 history) at step
    number step_start: lam_out = (∂b(step_start+nsteps)/∂b(step_start))ᵀ lam_in.
    The trajectory is recomputed in FP64 from checkpoints taken every
-   `checkpoint_every` steps; the context's b is left at the start state.
-   The step is timestep! (src/timestep.jl:195-242, :267-285) with φₛ
-   re-solved per stage. */
+   `checkpoint_every` steps; the context's b and AB2 history are left at the
+   start state.  The step is timestep! (src/timestep.jl:152-242, :267-285)
+   with φₛ re-solved before each tendency. */
 int ssg_adjoint_coupled(ssg_ctx* ctx, const ssg_time_params* params, const ssg_ma_params* ma, int64_t step_start,
                         int64_t nsteps, int32_t checkpoint_every, const double* lam_in, double* lam_out);
 

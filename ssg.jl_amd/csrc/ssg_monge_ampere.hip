@@ -830,51 +830,139 @@ int adjoint_coupled_sweep(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_par
     return SSG_OK;
 }
 
+// One coupled AB2 step in FP64 from (c->b, the held tendency), recording the
+// step's buoyancy and Monge-Ampère solution in slot 0 (src/timestep.jl:152-190
+// with φₛ re-solved before the tendency).
+int coupled_step_record_ab2(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* mp, int64_t step_after) {
+    const size_t n2 = (size_t)c->nx * c->ny;
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    ssg_ma_diag d{};
+    int rc;
+    HIP_OK(c, hipMemcpyAsync(c->adjc_bst[0], c->b, n2 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = ma_solve_dev(c, c->adjc_bst[0], mp, &d))) return rc;
+    if (!d.converged)
+        return set_err(c, SSG_ERR_NOT_CONVERGED, "Monge-Ampere solve did not converge (rel. residual %.3e)",
+                       d.relative_residual);
+    HIP_OK(c, hipMemcpyAsync(c->adjc_phis[0], c->ma_phi, c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice,
+                             c->stream));
+    if ((rc = flux_forward<double>(c, c->ncols_tend, c->adjc_bst[0]))) return rc;
+    if ((rc = tendency_cols<double>(c, 0))) return rc;
+    if ((rc = stage_inverse<double, EPI_AB2>(c, p->dt, 0, (step_after - 1) == 0 ? 1 : 0))) return rc;
+    if (p->filter_freq > 0 && (step_after % p->filter_freq) == 0)
+        return filter_b<double>(c, p->filter_strength, p->filter_order, p->filter_cutoff, 0);
+    return SSG_OK;
+}
+
+// Reverse sweep for coupled AB2 with adjoint state (λ_b, λ_f = c->adj_acc):
+// s = c λ_b + λ_f (c = 1.5 dt, or dt on the Euler step), λ_f <- -0.5 dt λ_b
+// (0 on the Euler step), λ_b <- λ_b + J(b_{n-1})ᵀ s, where J is the
+// linearised coupled tendency at the step's starting state.
+template <typename StateAt>
+int adjoint_coupled_sweep_ab2(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* mp, int64_t step0,
+                              int64_t nsteps, StateAt&& state_at) {
+    const double dt = p->dt;
+    const size_t n2 = (size_t)c->nx * c->ny;
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    ssg_ma_diag d{};
+    int rc;
+    for (int64_t j = nsteps; j >= 1; --j) {
+        const int64_t n = step0 + j;
+        if (p->filter_freq > 0 && (n % p->filter_freq) == 0)
+            if ((rc = filter_b<double>(c, p->filter_strength, p->filter_order, p->filter_cutoff, 0, c->adj_lam)))
+                return rc;
+        if ((rc = state_at(j - 1))) return rc;
+        HIP_OK(c, hipMemcpyAsync(c->adjc_bst[0], c->b, n2 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        if ((rc = ma_solve_dev(c, c->adjc_bst[0], mp, &d))) return rc;
+        if (!d.converged)
+            return set_err(c, SSG_ERR_NOT_CONVERGED, "Monge-Ampere solve did not converge (rel. residual %.3e)",
+                           d.relative_residual);
+        HIP_OK(c, hipMemcpyAsync(c->adjc_phis[0], c->ma_phi, c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice,
+                                 c->stream));
+        const bool euler = (n - 1) == 0;
+        if ((rc = lincomb(c, c->adj_s, euler ? dt : 1.5 * dt, c->adj_lam, 1.0, c->adj_acc))) return rc;
+        if ((rc = adjc_stage(c, mp, c->adj_s, 0))) return rc;
+        if ((rc = lincomb(c, c->adj_acc, euler ? 0.0 : -0.5 * dt, c->adj_lam, 0.0, nullptr))) return rc;
+        if ((rc = lincomb(c, c->adj_lam, 1.0, c->adj_lam, 1.0, c->adjc_g))) return rc;
+    }
+    return SSG_OK;
+}
+
 // checkpointed adjoint of nsteps coupled steps from the context's current b
 // (at step number step0): checkpoints every `every` steps on the forward
 // pass, then each segment is recomputed from its checkpoint, storing the
-// per-step states, and swept in reverse.
+// per-step states, and swept in reverse.  AB2 checkpoints also hold the
+// previous tendency, which the recomputation needs.  The context's b and
+// AB2 history are left at the start state.
 int adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* mp, int64_t step0, int64_t nsteps,
                     int every) {
+    const bool ab2 = p->scheme == SSG_AB2_LOWSTORAGE;
     const size_t n2 = (size_t)c->nx * c->ny, bytes = n2 * sizeof(double);
     const int64_t nck = (nsteps + every - 1) / every;
+    const int per = ab2 ? 2 : 1;  // checkpoint fields: b (and the held tendency)
     double* ck = nullptr;
     double* seg = nullptr;
-    HIP_OK(c, hipMalloc(&ck, (size_t)nck * bytes));
+    HIP_OK(c, hipMalloc(&ck, (size_t)nck * per * bytes));
     if (hipMalloc(&seg, (size_t)every * bytes) != hipSuccess) {
         (void)hipFree(ck);
         return set_err(c, SSG_ERR_HIP, "checkpoint allocation failed");
     }
+    const bool had_history = c->ab2_have_history;
+    auto step = [&](int64_t step_after) -> int {
+        return ab2 ? coupled_step_record_ab2(c, p, mp, step_after) : coupled_step_record(c, p, mp, step_after);
+    };
+    auto save_ck = [&](int64_t k) -> int {
+        HIP_OK(c, hipMemcpyAsync(ck + (size_t)k * per * n2, c->b, bytes, hipMemcpyDeviceToDevice, c->stream));
+        if (ab2)
+            HIP_OK(c, hipMemcpyAsync(ck + ((size_t)k * per + 1) * n2, c->d.fold, bytes, hipMemcpyDeviceToDevice,
+                                     c->stream));
+        return SSG_OK;
+    };
+    auto load_ck = [&](int64_t k) -> int {
+        HIP_OK(c, hipMemcpyAsync(c->b, ck + (size_t)k * per * n2, bytes, hipMemcpyDeviceToDevice, c->stream));
+        if (ab2)
+            HIP_OK(c, hipMemcpyAsync(c->d.fold, ck + ((size_t)k * per + 1) * n2, bytes, hipMemcpyDeviceToDevice,
+                                     c->stream));
+        return SSG_OK;
+    };
     auto body = [&]() -> int {
         int rc;
-        // forward pass with checkpoints; c->b is restored at the end
+        if (ab2) {
+            HIP_OK(c, hipMemsetAsync(c->adj_acc, 0, bytes, c->stream));  // λ_f
+            // the FP64 recomputation starts from the held tendency, which an
+            // FP32-mixed context keeps in FP32
+            if (c->precision == SSG_FP32_MIXED) {
+                hipLaunchKernelGGL((k_convert<float, double>), dim3(grid1d((long)n2)), dim3(256), 0, c->stream,
+                                   c->f.fold, c->d.fold, (long)n2);
+                HIP_OK(c, hipGetLastError());
+            }
+        }
         for (int64_t j = 0; j < nsteps; ++j) {
-            if (j % every == 0)
-                HIP_OK(c, hipMemcpyAsync(ck + (size_t)(j / every) * n2, c->b, bytes, hipMemcpyDeviceToDevice,
-                                         c->stream));
-            if ((rc = coupled_step_record(c, p, mp, step0 + j + 1))) return rc;
+            if (j % every == 0 && (rc = save_ck(j / every))) return rc;
+            if ((rc = step(step0 + j + 1))) return rc;
         }
         for (int64_t sg = nck - 1; sg >= 0; --sg) {
             const int64_t j0 = sg * every;
             const int64_t len = std::min<int64_t>(every, nsteps - j0);
             // recompute the segment's per-step states from its checkpoint
-            HIP_OK(c, hipMemcpyAsync(c->b, ck + (size_t)sg * n2, bytes, hipMemcpyDeviceToDevice, c->stream));
+            if ((rc = load_ck(sg))) return rc;
             for (int64_t k = 0; k < len; ++k) {
                 HIP_OK(c, hipMemcpyAsync(seg + (size_t)k * n2, c->b, bytes, hipMemcpyDeviceToDevice, c->stream));
-                if (k + 1 < len && (rc = coupled_step_record(c, p, mp, step0 + j0 + k + 1))) return rc;
+                if (k + 1 < len && (rc = step(step0 + j0 + k + 1))) return rc;
             }
             auto state_at = [&](int64_t jj) -> int {
                 HIP_OK(c, hipMemcpyAsync(c->b, seg + (size_t)jj * n2, bytes, hipMemcpyDeviceToDevice, c->stream));
                 return SSG_OK;
             };
-            if ((rc = adjoint_coupled_sweep(c, p, mp, step0 + j0, len, state_at))) return rc;
+            rc = ab2 ? adjoint_coupled_sweep_ab2(c, p, mp, step0 + j0, len, state_at)
+                     : adjoint_coupled_sweep(c, p, mp, step0 + j0, len, state_at);
+            if (rc) return rc;
         }
-        // leave b at the start state
-        HIP_OK(c, hipMemcpyAsync(c->b, ck, bytes, hipMemcpyDeviceToDevice, c->stream));
-        return SSG_OK;
+        // leave b (and the AB2 history) at the start state
+        return load_ck(0);
     };
-    const int rc = body();
+    const int rc = nsteps > 0 ? body() : SSG_OK;
     (void)hipStreamSynchronize(c->stream);
+    c->ab2_have_history = had_history;
     (void)hipFree(ck);
     (void)hipFree(seg);
     return rc;
@@ -1013,8 +1101,14 @@ int ssg_adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_param
                         int64_t nsteps, int32_t checkpoint_every, const double* lam_in, double* lam_out) {
     if (!c) return SSG_ERR_ARG;
     if (!p || !mp || !lam_in || !lam_out) return set_err(c, SSG_ERR_ARG, "null pointer");
-    if (p->scheme != SSG_RK3) return set_err(c, SSG_ERR_UNSUPPORTED, "the adjoint covers the RK3 scheme");
+    if (p->scheme != SSG_RK3 && p->scheme != SSG_AB2_LOWSTORAGE)
+        return set_err(c, SSG_ERR_UNSUPPORTED, "the adjoint covers the RK3 and AB2 schemes");
+    if (p->scheme == SSG_AB2_LOWSTORAGE && p->ab2_reference_aliasing)
+        return set_err(c, SSG_ERR_UNSUPPORTED, "the coupled AB2 adjoint uses the corrected tendency");
     if (nsteps < 0 || step_start < 0) return set_err(c, SSG_ERR_ARG, "need nsteps >= 0 and step_start >= 0");
+    if (p->scheme == SSG_AB2_LOWSTORAGE && step_start > 0 && !c->ab2_have_history)
+        return set_err(c, SSG_ERR_ARG, "AB2 step %lld needs the previous tendency; none is held",
+                       (long long)step_start);
     if (checkpoint_every < 1) return set_err(c, SSG_ERR_ARG, "checkpoint_every must be >= 1");
     if (!(p->dt > 0)) return set_err(c, SSG_ERR_ARG, "dt must be > 0");
     int rc = ma_check_params(c, mp);

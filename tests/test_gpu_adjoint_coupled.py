@@ -75,12 +75,12 @@ def coupled_forward(gpu, d, f, b0, params, nsteps, eps, variant):
     return f.bs
 
 
-def coupled_gradient_check(gpu, n, nz, nsteps, dt, every, ndir, filter_freq, seed_dirs):
+def coupled_gradient_check(gpu, n, nz, nsteps, dt, every, ndir, filter_freq, seed_dirs, scheme=1):
     eps = 0.1
     b0 = smooth(n, 0, k0=6.0)
     b_tgt = smooth(n, 1, k0=6.0)
     dA = (2 * np.pi / n) ** 2
-    params = gpu.TimeParams(dt, scheme=gpu.RK3, filter_freq=filter_freq, filter_strength=36.0, filter_order=8,
+    params = gpu.TimeParams(dt, scheme=scheme, filter_freq=filter_freq, filter_strength=36.0, filter_order=8,
                             filter_cutoff=0.65)
     d = gpu.make_domain(n, n, nz)
     f = gpu.allocate_fields(d)
@@ -112,6 +112,13 @@ def test_coupled_adjoint_small_with_segments(gpu):
     assert worst <= 1e-6
 
 
+def test_coupled_adjoint_ab2_with_segments(gpu):
+    """Coupled AB2 (forward Euler first step), 64²×6, 7 steps, checkpoints
+    every 3 carrying the held tendency, filter every 2."""
+    worst = coupled_gradient_check(gpu, 64, 6, 7, 2e-3, 3, 3, 2, 6, scheme=0)
+    assert worst <= 1e-6
+
+
 def test_coupled_adjoint_configuration(gpu):
     """The adjoint configuration with the Monge-Ampère solve in the step:
     256²×16, ε = 0.1, RK3 dt = 1e-3, 100 steps, checkpoint every 10 steps
@@ -124,7 +131,10 @@ def test_coupled_adjoint_configuration(gpu):
 def test_coupled_adjoint_rejects(gpu):
     d = gpu.make_domain(32, 32, 4)
     with pytest.raises(gpu.SSGError):
-        gpu.timestep_adjoint_coupled(d, gpu.TimeParams(1e-3, scheme=gpu.AB2_LowStorage), 0, 2, np.zeros((32, 32)))
+        gpu.timestep_adjoint_coupled(d, gpu.TimeParams(1e-3, scheme=gpu.RK3_LowStorage), 0, 2, np.zeros((32, 32)))
+    with pytest.raises(gpu.SSGError):   # the aliased AB2 tendency is not covered in coupled mode
+        gpu.timestep_adjoint_coupled(d, gpu.TimeParams(1e-3, scheme=gpu.AB2_LowStorage, ab2_reference_aliasing=True),
+                                     0, 2, np.zeros((32, 32)))
     with pytest.raises(gpu.SSGError):
         gpu.timestep_adjoint_coupled(d, gpu.TimeParams(1e-3, scheme=gpu.RK3), 0, 2, np.zeros((32, 32)),
                                      checkpoint_every=0)
