@@ -28,6 +28,17 @@ struct ssg_ensemble {
     int2* d_wrows = nullptr;
     int ncb = 0, nrb = 0;
     size_t lds_cols = 0, lds_rows = 0;
+    // batched filter (apply_spectral_filter! on every member at once): row
+    // transform of b, filter column pass, inverse rows with the update
+    // ([0] refreshing the next step's fluxes, [1] not)
+    void* d_ffwd = nullptr;
+    void* d_fcols = nullptr;
+    void* d_frows[2] = {nullptr, nullptr};
+    int2* d_wffwd = nullptr;
+    int2* d_wfcols = nullptr;
+    int nffb = 0, nfcb = 0;
+    size_t lds_ffwd = 0, lds_fcols = 0;
+    double built_filter[3] = {0.0, 0.0, 0.0};
     int built_ept = 0;
     std::vector<double> built_dt;
     hipGraphExec_t graph = nullptr;
@@ -65,6 +76,14 @@ void ens_free_tables(ssg_ensemble* e) {
     }
     (void)hipFree(e->d_wcols);
     (void)hipFree(e->d_wrows);
+    (void)hipFree(e->d_ffwd);
+    (void)hipFree(e->d_fcols);
+    (void)hipFree(e->d_frows[0]);
+    (void)hipFree(e->d_frows[1]);
+    (void)hipFree(e->d_wffwd);
+    (void)hipFree(e->d_wfcols);
+    e->d_ffwd = e->d_fcols = e->d_frows[0] = e->d_frows[1] = nullptr;
+    e->d_wffwd = e->d_wfcols = nullptr;
     e->d_cols = nullptr;
     e->d_wcols = e->d_wrows = nullptr;
     e->built_ept = 0;
@@ -72,9 +91,16 @@ void ens_free_tables(ssg_ensemble* e) {
 
 // per-member argument structs and the block tables for elements-per-thread E
 template <typename R>
-int ens_build(ssg_ensemble* e, int E, const double* dt) {
+int ens_build(ssg_ensemble* e, int E, const double* dt, const ssg_time_params* p) {
     ens_free_tables(e);
     std::vector<ColsArgs<R>> ca(e->n);
+    std::vector<RowsFwdArgs<R, double>> fa(e->n);
+    std::vector<ColsArgs<R>> fc(e->n);
+    std::vector<RowsInvArgs<R>> fr[2];
+    for (auto& v : fr) v.resize(e->n);
+    std::vector<int2> wf, wfc;
+    size_t lf = 0, lfc = 0;
+    const int delta = std::is_same<R, float>::value ? 1 : 0;
     std::vector<RowsInvArgs<R>> ra[8];
     for (auto& v : ra) v.resize(e->n);
     std::vector<int2> wc, wr;
@@ -123,7 +149,66 @@ int ens_build(ssg_ensemble* e, int E, const double* dt) {
         }
         for (int j = 0; j < c->ny / (2 * Gr); ++j) wr.push_back(int2{i, j});
         lr = std::max(lr, (size_t)Gr * 2 * lds_line_elems(c->nx) * sizeof(cpx<R>));
+        // filter: the same launch shapes as filter_b
+        RowsFwdArgs<R, double> f{c->b, nullptr, B.Tp, nullptr, c->ncols_full, Gr, c->ny / (2 * Gr), B.twx, c->g};
+        fa[i] = f;
+        for (int j = 0; j < c->ny / (2 * Gr); ++j) wf.push_back(int2{i, j});
+        lf = std::max(lf, (size_t)Gr * lds_line_elems(c->nx) * sizeof(cpx<R>));
+        ColsArgs<R> q{};
+        q.inA = B.Tp;
+        q.outA = B.Tp;
+        q.ncols = c->ncols_full;
+        q.G = Gc;
+        q.scale = (R)(1.0 / ((double)c->nx * c->ny));
+        q.fstrength = (R)p->filter_strength;
+        q.forder = p->filter_order;
+        q.fcutoff = (R)p->filter_cutoff;
+        q.fdelta = delta;
+        const double kmx = M_PI * c->nx / c->lx, kmy = M_PI * c->ny / c->ly;
+        q.inv_kmax = (R)(1.0 / std::sqrt(kmx * kmx + kmy * kmy));
+        q.twy = B.twy;
+        q.g = c->g;
+        q.nlev = 1;
+        fc[i] = q;
+        const int nfc = (c->ncols_full + Gc - 1) / Gc;
+        for (int j = 0; j < nfc; ++j) wfc.push_back(int2{i, j});
+        lfc = std::max(lfc, (size_t)Gc * lds_line_elems(c->ny) * sizeof(cpx<R>));
+        for (int k = 0; k < 2; ++k) {
+            RowsInvArgs<R> r{};
+            r.P0 = B.Tp;
+            r.ncols_in = c->ncols_full;
+            r.b = c->b;
+            r.emit = k == 0 ? 1 : 0;
+            r.w = B.w;
+            r.Fx = B.Fx;
+            r.Fy = B.Fy;
+            r.ncols_out = c->ncols_tend;
+            r.G = Gr;
+            r.nblk = c->ny / (2 * Gr);
+            r.twx = B.twx;
+            r.g = c->g;
+            fr[k][i] = r;
+        }
     }
+    auto upload = [&](void** dst, const void* src, size_t bytes) -> int {
+        ENS_OK(e, hipMalloc(dst, bytes));
+        ENS_OK(e, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return SSG_OK;
+    };
+    int urc;
+    if ((urc = upload(&e->d_ffwd, fa.data(), fa.size() * sizeof(fa[0])))) return urc;
+    if ((urc = upload(&e->d_fcols, fc.data(), fc.size() * sizeof(fc[0])))) return urc;
+    for (int k = 0; k < 2; ++k)
+        if ((urc = upload(&e->d_frows[k], fr[k].data(), fr[k].size() * sizeof(fr[k][0])))) return urc;
+    if ((urc = upload((void**)&e->d_wffwd, wf.data(), wf.size() * sizeof(int2)))) return urc;
+    if ((urc = upload((void**)&e->d_wfcols, wfc.data(), wfc.size() * sizeof(int2)))) return urc;
+    e->nffb = (int)wf.size();
+    e->nfcb = (int)wfc.size();
+    e->lds_ffwd = lf;
+    e->lds_fcols = lfc;
+    e->built_filter[0] = p->filter_strength;
+    e->built_filter[1] = p->filter_order;
+    e->built_filter[2] = p->filter_cutoff;
     ENS_OK(e, hipMalloc(&e->d_cols, ca.size() * sizeof(ColsArgs<R>)));
     ENS_OK(e, hipMemcpy(e->d_cols, ca.data(), ca.size() * sizeof(ColsArgs<R>), hipMemcpyHostToDevice));
     for (int k = 0; k < 8; ++k) {
@@ -189,6 +274,48 @@ int ens_stage(ssg_ensemble* e, int stage_k) {
     }
 }
 
+// apply_spectral_filter! on every member: three launches for the ensemble
+template <typename R, int E>
+int ens_launch_filter(ssg_ensemble* e, int emit) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)k_rows_fwd_batch<R, E, SRC_PAIR>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)k_cols_batch<R, E, OP_FILT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void*)k_rows_inv_batch<R, E, EPI_ADD>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)k_rows_inv_batch<R, E, EPI_SET>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_rows_fwd_batch<R, E, SRC_PAIR>), dim3(e->nffb), dim3(256), e->lds_ffwd, e->stream,
+                       (const RowsFwdArgs<R, double>*)e->d_ffwd, e->d_wffwd);
+    ENS_OK(e, hipGetLastError());
+    hipLaunchKernelGGL((k_cols_batch<R, E, OP_FILT>), dim3(e->nfcb), dim3(256), e->lds_fcols, e->stream,
+                       (const ColsArgs<R>*)e->d_fcols, e->d_wfcols);
+    ENS_OK(e, hipGetLastError());
+    const RowsInvArgs<R>* ra = (const RowsInvArgs<R>*)e->d_frows[emit ? 0 : 1];
+    if (std::is_same<R, float>::value)
+        hipLaunchKernelGGL((k_rows_inv_batch<R, E, EPI_ADD>), dim3(e->nrb), dim3(256), e->lds_rows, e->stream, ra,
+                           e->d_wrows);
+    else
+        hipLaunchKernelGGL((k_rows_inv_batch<R, E, EPI_SET>), dim3(e->nrb), dim3(256), e->lds_rows, e->stream, ra,
+                           e->d_wrows);
+    ENS_OK(e, hipGetLastError());
+    return SSG_OK;
+}
+
+template <typename R>
+int ens_filter(ssg_ensemble* e, int emit) {
+    switch (e->built_ept) {
+        case 4: return ens_launch_filter<R, 4>(e, emit);
+        case 8: return ens_launch_filter<R, 8>(e, emit);
+        case 16: return ens_launch_filter<R, 16>(e, emit);
+        default: return ens_err(e, SSG_ERR_ARG, "ensemble tables not built");
+    }
+}
+
 // point every member's launches at the ensemble stream for the duration
 struct StreamSwap {
     ssg_ensemble* e;
@@ -219,12 +346,7 @@ int ens_issue_step(ssg_ensemble* e, const ssg_time_params* p, int64_t step_befor
         if ((rc = ens_stage<R>(e, 1))) return rc;
         if ((rc = ens_stage<R>(e, emit ? 2 : 3))) return rc;
     }
-    if (filt) {
-        for (ssg_ctx* c : e->m) {
-            if ((rc = filter_b<R>(c, p->filter_strength, p->filter_order, p->filter_cutoff, last ? 0 : 1)))
-                return ens_err(e, rc, "member filter: %s", c->err);
-        }
-    }
+    if (filt && (rc = ens_filter<R>(e, last ? 0 : 1))) return rc;
     return SSG_OK;
 }
 
@@ -240,7 +362,7 @@ int ens_prologue(ssg_ensemble* e) {
 // autotuned tile shape: elements per thread of the batched launches, timed
 // over one full RK3 step (6 launches); the probe only writes scratch planes
 template <typename R>
-int ens_tune(ssg_ensemble* e, const double* dt) {
+int ens_tune(ssg_ensemble* e, const double* dt, const ssg_time_params* p) {
     if (e->tuned) return SSG_OK;
     e->tuned = true;
     if (const char* env = getenv("SSG_ENS_EPT")) {
@@ -255,7 +377,7 @@ int ens_tune(ssg_ensemble* e, const double* dt) {
     int rc = SSG_OK;
     for (int i = 0; i < 3 && rc == SSG_OK; ++i) {
         e->tune_us[i] = -1.0;
-        if ((rc = ens_build<R>(e, cand[i], dt))) {
+        if ((rc = ens_build<R>(e, cand[i], dt, p))) {
             rc = SSG_OK;  // shape does not fit every member: skip
             continue;
         }
@@ -283,14 +405,16 @@ template <typename R>
 int ens_run(ssg_ensemble* e, const ssg_time_params* p, const double* dt, ssg_time_state* st, int64_t nsteps) {
     StreamSwap swap(e);
     int rc;
-    if ((rc = ens_tune<R>(e, dt))) return rc;
+    if ((rc = ens_tune<R>(e, dt, p))) return rc;
     const bool same_dt = e->built_dt.size() == (size_t)e->n && std::equal(dt, dt + e->n, e->built_dt.begin());
-    if (e->built_ept != e->ept || !same_dt) {
+    const bool same_filter = e->built_filter[0] == p->filter_strength && e->built_filter[1] == p->filter_order &&
+                             e->built_filter[2] == p->filter_cutoff;
+    if (e->built_ept != e->ept || !same_dt || !same_filter) {
         if (e->graph) {
             (void)hipGraphExecDestroy(e->graph);
             e->graph = nullptr;
         }
-        if ((rc = ens_build<R>(e, e->ept, dt))) return rc;
+        if ((rc = ens_build<R>(e, e->ept, dt, p))) return rc;
     }
     if ((rc = ens_prologue<R>(e))) return rc;
     const char* genv = getenv("SSG_GRAPH");

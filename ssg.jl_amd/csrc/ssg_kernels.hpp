@@ -979,6 +979,20 @@ __global__ __launch_bounds__(256) void k_cols_batch(const ColsArgs<R>* __restric
     }
 }
 
+template <typename R, int E, int SRC>
+__global__ __launch_bounds__(256) void k_rows_fwd_batch(const RowsFwdArgs<R, double>* __restrict__ args,
+                                                        const int2* __restrict__ work) {
+    const int2 wk = work[blockIdx.x];
+    const RowsFwdArgs<R, double>& a = args[wk.x];
+    switch (a.g.nx) {
+        case 128: rows_fwd_body<R, double, 128, E, SRC>(a, wk.y); break;
+        case 256: rows_fwd_body<R, double, 256, E, SRC>(a, wk.y); break;
+        case 512: rows_fwd_body<R, double, 512, E, SRC>(a, wk.y); break;
+        case 1024: rows_fwd_body<R, double, 1024, E, SRC>(a, wk.y); break;
+        default: break;
+    }
+}
+
 template <typename R, int E, int EPI>
 __global__ __launch_bounds__(256) void k_rows_inv_batch(const RowsInvArgs<R>* __restrict__ args,
                                                         const int2* __restrict__ work) {
