@@ -99,7 +99,7 @@ def cpu_baseline(b0, psi, nx, ny, nz, dt, budget_s):
             "sample": f"{n} RK3 steps of the same {nx}x{ny} workload (FP64 C oracle, OpenMP), {el:.1f} s"}
 
 
-def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec):
+def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2):
     """Coupled step (Monge-Ampère solve at every RK stage, body-forced
     variant, ε = 0.1, relative residual 1e-10): steps/s plus the Monge-Ampère
     solve's iterations, time and per-kernel-class profile, and the final
@@ -125,7 +125,7 @@ def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec):
     # restart from the cold state so both precisions step the same trajectory
     f.bs = b0
     ssg.solve_monge_ampere(f, d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY)
-    ssg.set_coupling(d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY, warm=True)
+    ssg.set_coupling(d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY, warm=warm)
     st = ssg.TimeState(0.0, f)
     params = ssg.TimeParams(dt, scheme=ssg.RK3, filter_freq=10, filter_strength=36.0, filter_order=8,
                             filter_cutoff=0.65)
@@ -140,7 +140,7 @@ def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec):
     fields = {"b": f.bs, "phis": f.phis, "ke": ssg.diagnostics(d)["ke_surface"]}
     d.close()
     return {"steps_per_s": steps / wall, "steps": steps, "variant": "body (as wired)", "eps": 0.1, "tol": 1e-10,
-            "cold_solve": cold, "warm_solve_last_iterations": last.iterations,
+            "cold_solve": cold, "warm_start": warm, "warm_solve_last_iterations": last.iterations,
             "warm_solve_last_iterations_fp32": last.iterations_fp32,
             "warm_solve_last_relative_residual": last.relative_residual, "ma_kernels": kernels}, fields
 
@@ -341,6 +341,8 @@ def main():
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None)
     ap.add_argument("--coupled-steps", type=int, default=3,
                     help="also time this many coupled steps (Monge-Ampère solve per stage); 0 = skip")
+    ap.add_argument("--ma-warm", type=int, default=2, choices=(0, 1, 2),
+                    help="warm start of the coupled step's solves: 0 cold, 1 last D(Φ), 2 extrapolated")
     ap.add_argument("--ab2-steps", type=int, default=1000,
                     help="also time this many AB2 steps per precision; 0 = skip")
     ap.add_argument("--adjoint-steps", type=int, default=100,
@@ -433,8 +435,9 @@ def main():
         out["ab2"] = measure_ab2(ssg, b0, psi, nx, ny, nz, args.dt, args.ab2_steps)
     if args.coupled_steps > 0 and not args.only:
         log(f"bench: coupled RK3 {nx}x{ny}x{nz}, {args.coupled_steps} steps per precision")
-        c64, f64 = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP64)
-        c32, f32 = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP32_MIXED)
+        c64, f64 = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP64, args.ma_warm)
+        c32, f32 = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP32_MIXED,
+                                   args.ma_warm)
         out["coupled_fp64"] = c64
         out["coupled_fp32"] = c32
         out["coupled_fp32_over_fp64"] = c32["steps_per_s"] / c64["steps_per_s"]

@@ -107,7 +107,9 @@ typedef struct {
     double tol;        /* relative residual target ‖r‖₂ / ‖b + εD‖₂ on interior levels */
     int32_t maxiter;   /* src/poisson.jl:874 default 50 */
     int32_t variant;   /* SSG_MA_BODY or SSG_MA_SURFACE */
-    int32_t warm;      /* 1: start from the previous solve's D(Φ) */
+    int32_t warm;      /* 0: cold start; 1: start from the previous solve's D(Φ); 2: from the linear
+                          extrapolation 2 D̂_k - D̂_{k-1} of the last two solves (for equally spaced
+                          solves: the RK3 stages, abscissae 0, 1/3, 2/3, and AB2 steps) */
 } ssg_ma_params;
 
 /* solve_ssg_equation diagnostics (src/poisson.jl:928-958) */

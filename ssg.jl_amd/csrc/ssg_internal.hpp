@@ -80,6 +80,9 @@ struct ssg_ctx {
     int ma_partial_cap = 0;
     int ma_partial_used = 0;
     bool ma_have_d = false;
+    cpx<double>* ma_dprev2 = nullptr; // D̂ of the solve before the last (extrapolated warm starts)
+    int ma_hist = 0;                  // consecutive solves whose D̂ is held (0, 1, 2)
+    double ma_last_r0 = 1e300;        // first relative residual of the last solve
     // FP32 phase of the mixed-precision solve: Φ̂ and D̂ in FP32; the three
     // derivative planes alias ma_p (an FP64 plane holds two FP32 planes)
     cpx<float>* ma32_phi = nullptr;

@@ -928,6 +928,15 @@ static __global__ __launch_bounds__(256) void k_wmax(const double2* __restrict__
     }
 }
 
+// warm-start extrapolation over equally spaced solves: d2 <- d, d <- 2 d - d2
+static __global__ void k_extrap(double2* __restrict__ d, double2* __restrict__ d2, long n) {
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const double2 a = d[q], b = d2[q];
+        d2[q] = a;
+        d[q] = double2{2.0 * a.x - b.x, 2.0 * a.y - b.y};
+    }
+}
+
 // out = a x + b y (y may be null)
 // (out may alias x or y: each element is read before it is written)
 static __global__ void k_lincomb(double* out, double ca, const double* x, double cb, const double* y, long n) {

@@ -65,6 +65,7 @@ void ma_free(ssg_ctx* c) {
     (void)hipFree(c->ma_b2);
     (void)hipFree(c->ma32_phi);
     (void)hipFree(c->ma32_d);
+    (void)hipFree(c->ma_dprev2);
     (void)hipFree(c->ma_coef);
     (void)hipFree(c->ma_partial);
     if (c->ma_partial_host) (void)hipHostFree(c->ma_partial_host);
@@ -276,6 +277,21 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
     if (rc0 == SSG_OK && c->precision == SSG_FP32_MIXED) rc0 = ma_tune<float>(c);
     if (rc0) return rc0;
     const bool warm = p->warm && c->ma_have_d;
+    if (warm && p->warm >= 2) {
+        // extrapolated warm start: with two held solves the first iterate
+        // starts from 2 D̂_k - D̂_{k-1}; the residual stays exact, since the
+        // iterate is L⁻¹ of whatever D̂ it was built from
+        if (!c->ma_dprev2) HIP_OK(c, hipMalloc(&c->ma_dprev2, c3 * sizeof(cpx<double>)));
+        if (c->ma_hist >= 2) {
+            hipLaunchKernelGGL(k_extrap, dim3(grid1d((long)c3)), dim3(256), 0, c->stream, c->ma_dprev, c->ma_dprev2,
+                               (long)c3);
+            c->launches++;
+            HIP_OK(c, hipGetLastError());
+        } else {
+            HIP_OK(c, hipMemcpyAsync(c->ma_dprev2, c->ma_dprev, c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice,
+                                     c->stream));
+        }
+    }
     memset(d, 0, sizeof(*d));
     int it = 0, rc;
     double best = 1e300;
@@ -298,7 +314,11 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
         }
         return it > p->maxiter;
     };
-    if (c->precision == SSG_FP32_MIXED) {
+    // A warm solve whose predecessor already started near the FP32 floor
+    // skips the FP32 phase: it could not lower the residual, and its passes
+    // and precision switch would only add to the FP64 iterations
+    const bool phase32 = c->precision == SSG_FP32_MIXED && !(warm && c->ma_last_r0 <= 10.0 * kMaFp32Floor);
+    if (phase32) {
         cpx<float>* const pl[3] = {(cpx<float>*)c->ma_p[0], (cpx<float>*)c->ma_p[1], (cpx<float>*)c->ma_p[2]};
         if (warm) {
             if ((rc = cpx_convert<double, float>(c, c->ma_dprev, c->ma32_d, c3))) return rc;
@@ -322,6 +342,8 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
         if ((rc = ma_tridiag<float, double>(c, p, 1, c->ma32_d, c->ma_phi))) return rc;
         if (stop && (!std::isfinite(rr) || rr > 100.0 * best)) {
             c->ma_have_d = false;
+            c->ma_hist = 0;
+            c->ma_last_r0 = 1e300;
             return SSG_OK;
         }
     } else {
@@ -335,6 +357,8 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
         if ((rc = ma_tridiag<double, double>(c, p, 1, c->ma_dprev, c->ma_phi))) return rc;
     }
     c->ma_have_d = true;
+    c->ma_hist = (warm && p->warm >= 2) ? 2 : 1;
+    c->ma_last_r0 = d->history[0];
     return SSG_OK;
 }
 
@@ -381,6 +405,7 @@ int ma_check_params(ssg_ctx* c, const ssg_ma_params* p) {
     if (p->maxiter < 0) return set_err(c, SSG_ERR_ARG, "maxiter must be >= 0");
     if (p->variant != SSG_MA_BODY && p->variant != SSG_MA_SURFACE)
         return set_err(c, SSG_ERR_ARG, "unknown Monge-Ampere variant %d", p->variant);
+    if (p->warm < 0 || p->warm > 2) return set_err(c, SSG_ERR_ARG, "warm must be 0, 1 or 2 (got %d)", p->warm);
     return SSG_OK;
 }
 

@@ -468,7 +468,7 @@ def diagnostics(domain: Domain) -> dict:
 
 
 def _ma_params(eps, tol, maxiter, variant, warm):
-    return _MaParams(float(eps), float(tol), int(maxiter), int(variant), int(bool(warm)))
+    return _MaParams(float(eps), float(tol), int(maxiter), int(variant), int(warm))
 
 
 def solve_monge_ampere(fields: Fields, domain: Domain, *, eps: float = 0.1, tol: float = 1e-10, maxiter: int = 50,
@@ -497,9 +497,11 @@ def get_phi3d(domain: Domain) -> np.ndarray:
 
 
 def set_coupling(domain: Domain, *, eps: float | None = 0.1, tol: float = 1e-10, maxiter: int = 50,
-                 variant: int = MA_BODY, warm: bool = True) -> None:
+                 variant: int = MA_BODY, warm: int = True) -> None:
     """Coupled stepping: re-solve the Monge-Ampère problem at every RK stage
-    (``eps=None`` restores the reference's frozen φₛ)."""
+    (``eps=None`` restores the reference's frozen φₛ).  ``warm``: 0 cold
+    starts, 1 starts each solve from the last D(Φ), 2 from the linear
+    extrapolation of the last two (the stage abscissae are equally spaced)."""
     if eps is None:
         domain.check(domain.lib.ssg_set_coupling(domain.handle, None))
         domain.coupled = False

@@ -8,6 +8,8 @@ independent dense Newton solve (oracle/ma_newton.py) and, at ε = 0, with
 closed forms.  The horizontal-mean (k = 0) profile is compared separately:
 its Neumann-Neumann problem is singular and is pinned by a gauge.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -147,15 +149,17 @@ def test_not_converged_returns_flag(gpu):
     assert not diag["converged"] and diag["iterations"] == 2
 
 
-def test_coupled_rk3_against_oracle(gpu):
+@pytest.mark.parametrize("warm", [1, 2])
+def test_coupled_rk3_against_oracle(gpu, warm):
     """Coupled step: the MA problem re-solved for every RK stage (φₛ follows b),
-    against the oracle tendency fed by dense-Newton solves."""
+    against the oracle tendency fed by dense-Newton solves (warm starts from
+    the last D(Φ) and from the extrapolation of the last two)."""
     n, nz, eps, dt = 8, 4, 0.1, 1e-2
     b0 = smooth_b(n, 5, 1.2)
     d = gpu.make_domain(n, n, nz)
     f = gpu.allocate_fields(d)
     f.bs = b0
-    gpu.set_coupling(d, eps=eps, tol=1e-13, maxiter=80, variant=gpu.MA_BODY)
+    gpu.set_coupling(d, eps=eps, tol=1e-13, maxiter=80, variant=gpu.MA_BODY, warm=warm)
     st = gpu.TimeState(0.0, f)
     gpu.timestep(f, d, gpu.TimeParams(dt, scheme=gpu.RK3, filter_freq=0), st, 2)
     g = f.bs
@@ -275,7 +279,8 @@ def test_rectangular_grids_newton(gpu, nx, ny, nz, variant, k0):
     d.close()
 
 
-def test_coupled_ab2_against_oracle(gpu):
+@pytest.mark.parametrize("warm", [1, 2])
+def test_coupled_ab2_against_oracle(gpu, warm):
     """Coupled AB2 (forward Euler first step): the Monge-Ampère problem
     re-solved before every tendency, against the oracle tendency fed by
     dense-Newton solves."""
@@ -284,7 +289,7 @@ def test_coupled_ab2_against_oracle(gpu):
     d = gpu.make_domain(n, n, nz)
     f = gpu.allocate_fields(d)
     f.bs = b0
-    gpu.set_coupling(d, eps=eps, tol=1e-13, maxiter=80, variant=gpu.MA_BODY)
+    gpu.set_coupling(d, eps=eps, tol=1e-13, maxiter=80, variant=gpu.MA_BODY, warm=warm)
     st = gpu.TimeState(0.0, f)
     gpu.timestep(f, d, gpu.TimeParams(dt, scheme=gpu.AB2_LowStorage, filter_freq=0), st, 3)
     g = f.bs
@@ -302,3 +307,33 @@ def test_coupled_ab2_against_oracle(gpu):
         fold = t
     assert np.abs(g - b).max() < 1e-9 * np.abs(b).max()
     d.close()
+
+
+@pytest.mark.parametrize("prec", ["SSG_FP64", "SSG_FP32_MIXED"])
+def test_extrapolated_warm_start(gpu, prec):
+    """Coupled RK3 at 128²×8 with warm=2 (first iterate from 2 D̂ₖ - D̂ₖ₋₁)
+    against warm=1: the same trajectory to the solve tolerance, and no more
+    iterations in the last solve."""
+    n, nz, eps, dt, steps = 128, 8, 0.1, 1e-3, 10
+    b0 = smooth_b(n, 6, 6.0)
+    out = {}
+    for warm in (1, 2):
+        d = gpu.make_domain(n, n, nz, precision=getattr(gpu, prec))
+        f = gpu.allocate_fields(d)
+        f.bs = b0
+        gpu.solve_monge_ampere(f, d, eps=eps, tol=1e-10, maxiter=60, variant=gpu.MA_BODY)
+        gpu.set_coupling(d, eps=eps, tol=1e-10, maxiter=60, variant=gpu.MA_BODY, warm=warm)
+        st = gpu.TimeState(0.0, f)
+        gpu.timestep(f, d, gpu.TimeParams(dt, scheme=gpu.RK3, filter_freq=5, filter_strength=36.0, filter_order=8,
+                                          filter_cutoff=0.65), st, steps)
+        last = gpu._MaDiag()
+        d.check(d.lib.ssg_last_ma_diag(d.handle, ctypes.byref(last)))
+        out[warm] = (f.bs, last.iterations, last.iterations_fp32, last.relative_residual, bool(last.converged))
+        d.close()
+    (b1, it1, f1, r1, c1), (b2, it2, f2, r2, c2) = out[1], out[2]
+    e = np.linalg.norm(b2 - b1) / np.linalg.norm(b1)
+    print(f"{prec}: warm=1 last solve {it1} its ({f1} fp32) r={r1:.1e}; warm=2 {it2} its ({f2} fp32) r={r2:.1e};"
+          f" b diff {e:.1e}", flush=True)
+    assert c1 and c2
+    assert e < 1e-8
+    assert it2 <= it1
