@@ -8,7 +8,8 @@ transposed tridiagonal solve, the boundary-row eliminations and the gauge
 of the mean mode); (2) at ε > 0 the vector-Jacobian product of the solve
 against central differences of converged GPU solves; (3) the gradient of
 J = ½∫(b(T) - b_tgt)² dA through coupled RK3 steps, with checkpointing,
-against central differences of GPU forward runs."""
+against central differences of GPU forward runs, and (4) the same gradient
+against the forward-mode derivative of the CPU oracle's coupled step."""
 import numpy as np
 import pytest
 
@@ -126,6 +127,47 @@ def test_coupled_adjoint_configuration(gpu):
     step 1e-6, FP64."""
     worst = coupled_gradient_check(gpu, 256, 16, 100, 1e-3, 10, 8, 10, 2)
     assert worst <= 1e-6
+
+
+def test_coupled_adjoint_against_cpu_forward_mode(gpu):
+    """The gradient through coupled RK3 steps against the forward-mode
+    reference derivative of the CPU oracle (the role ForwardDiff plays for
+    the Julia step): ⟨∇J, v⟩ = ∫(b(T) - b_tgt) δb(T; v) dA, with δb(T; v)
+    propagated through the oracle's Newton-Krylov Monge-Ampère tangents, its
+    bilinear tendency and its filter.  64²×6, 7 steps, filter every 2,
+    checkpoints every 3; three unit-RMS directions."""
+    from oracle import Oracle
+    from oracle.tangent import CoupledTangent
+    n, nz, nsteps, dt, eps = 64, 6, 7, 2e-3, 0.1
+    b0 = smooth(n, 0, k0=6.0)
+    b_tgt = smooth(n, 1, k0=6.0)
+    dA = (2 * np.pi / n) ** 2
+    params = gpu.TimeParams(dt, scheme=gpu.RK3, filter_freq=2, filter_strength=36.0, filter_order=8,
+                            filter_cutoff=0.65)
+    d = gpu.make_domain(n, n, nz)
+    f = gpu.allocate_fields(d)
+    bT = coupled_forward(gpu, d, f, b0, params, nsteps, eps, gpu.MA_BODY)
+    f.bs = b0
+    grad = gpu.timestep_adjoint_coupled(d, params, 0, nsteps, (bT - b_tgt) * dA, eps=eps, tol=1e-13, maxiter=200,
+                                        variant=gpu.MA_BODY, warm=True, checkpoint_every=3)
+    d.close()
+    rng = np.random.default_rng(11)
+    dirs = []
+    for _ in range(3):
+        v = rng.standard_normal((n, n))
+        dirs.append(v / np.sqrt(np.mean(v * v)))
+    ct = CoupledTangent(Oracle(n, n, nz), eps)
+    bT_cpu, dbs = ct.run(b0, dirs, dt, nsteps, filter_freq=2, filter_strength=36.0, filter_order=8,
+                         filter_cutoff=0.65)
+    fwd_err = np.abs(bT_cpu - bT).max() / np.abs(bT).max()
+    print(f"forward b(T), GPU vs CPU oracle: {fwd_err:.1e}", flush=True)
+    assert fwd_err < 1e-10
+    for k, (v, db) in enumerate(zip(dirs, dbs)):
+        ref = np.sum((bT_cpu - b_tgt) * db) * dA
+        an = np.sum(grad * v)
+        rel = abs(an - ref) / abs(ref)
+        print(f"direction {k}: adjoint {an:.14e} CPU forward mode {ref:.14e} rel {rel:.1e}", flush=True)
+        assert rel < 1e-11
 
 
 def test_coupled_adjoint_rejects(gpu):
