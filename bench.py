@@ -339,7 +339,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None)
-    ap.add_argument("--coupled-steps", type=int, default=3,
+    ap.add_argument("--coupled-steps", type=int, default=30,
                     help="also time this many coupled steps (Monge-Ampère solve per stage); 0 = skip")
     ap.add_argument("--ma-warm", type=int, default=2, choices=(0, 1, 2),
                     help="warm start of the coupled step's solves: 0 cold, 1 last D(Φ), 2 extrapolated")

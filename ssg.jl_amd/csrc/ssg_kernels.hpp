@@ -928,12 +928,11 @@ static __global__ __launch_bounds__(256) void k_wmax(const double2* __restrict__
     }
 }
 
-// warm-start extrapolation over equally spaced solves: d2 <- d, d <- 2 d - d2
-static __global__ void k_extrap(double2* __restrict__ d, double2* __restrict__ d2, long n) {
+// warm-start extrapolation over equally spaced solves: older <- 2 last - older
+static __global__ void k_extrap(const double2* __restrict__ last, double2* __restrict__ older, long n) {
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const double2 a = d[q], b = d2[q];
-        d2[q] = a;
-        d[q] = double2{2.0 * a.x - b.x, 2.0 * a.y - b.y};
+        const double2 a = last[q], b = older[q];
+        older[q] = double2{2.0 * a.x - b.x, 2.0 * a.y - b.y};
     }
 }
 

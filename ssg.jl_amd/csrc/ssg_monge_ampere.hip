@@ -280,13 +280,16 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
     if (warm && p->warm >= 2) {
         // extrapolated warm start: with two held solves the first iterate
         // starts from 2 D̂_k - D̂_{k-1}; the residual stays exact, since the
-        // iterate is L⁻¹ of whatever D̂ it was built from
+        // iterate is L⁻¹ of whatever D̂ it was built from.  The guess is
+        // written over D̂_{k-1} and the two buffers swap roles, so the solve
+        // iterates in that buffer while D̂_k stays held for the next solve.
         if (!c->ma_dprev2) HIP_OK(c, hipMalloc(&c->ma_dprev2, c3 * sizeof(cpx<double>)));
         if (c->ma_hist >= 2) {
             hipLaunchKernelGGL(k_extrap, dim3(grid1d((long)c3)), dim3(256), 0, c->stream, c->ma_dprev, c->ma_dprev2,
                                (long)c3);
             c->launches++;
             HIP_OK(c, hipGetLastError());
+            std::swap(c->ma_dprev, c->ma_dprev2);
         } else {
             HIP_OK(c, hipMemcpyAsync(c->ma_dprev2, c->ma_dprev, c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice,
                                      c->stream));
