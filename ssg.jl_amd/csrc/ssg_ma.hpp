@@ -248,11 +248,24 @@ __global__ __launch_bounds__(128) void k_ma_tridiag(MaTriArgs<R, P> a) {
     };
     double cp[NZMAX];
     double2 dp[NZMAX];
+    // Every right-hand side f_k (interior levels 1..nz-2) is loaded into
+    // dp[k] before the sweep: the loads are independent of the recurrence,
+    // so all of them are in flight at once instead of one per level.  The
+    // sweep then overwrites dp[k] in place (f_k is read before dp[k] is
+    // written); the top row still needs f_{nz-2}, kept in f_top.
+    double2 f_top = {0.0, 0.0};
+#pragma unroll
+    for (int k = 1; k < NZMAX; ++k) {
+        if (k < nz - 1) {
+            dp[k] = rhs_at(k);
+            if (k == nz - 2) f_top = dp[k];
+        }
+    }
     // row 0: bottom Neumann row with Φ2 eliminated through interior row 1
     {
         const double A0 = cf[3 * nz], B0 = cf[3 * nz + 1], C0 = cf[3 * nz + 2];
         const double al = cf[3], be = cf[4] + hs, ga = cf[5];
-        const double2 f1 = rhs_at(1);
+        const double2 f1 = dp[1];
         double a0 = A0 - C0 * al / ga, b0 = B0 - C0 * be / ga;
         double2 r0 = {-C0 * f1.x / ga, -C0 * f1.y / ga};
         if (kx == 0 && ky == 0) {  // gauge of the Neumann-Neumann mean mode
@@ -269,22 +282,22 @@ __global__ __launch_bounds__(128) void k_ma_tridiag(MaTriArgs<R, P> a) {
     for (int k = 1; k < NZMAX; ++k) {
         if (k < nz - 1) {
             const double al = cf[3 * k], be = cf[3 * k + 1] + hs, ga = cf[3 * k + 2];
-            const double2 f = rhs_at(k);
-            const double m = be - al * cp[k - 1];
-            cp[k] = ga / m;
-            dp[k] = {(f.x - al * dp[k - 1].x) / m, (f.y - al * dp[k - 1].y) / m};
+            const double2 f = dp[k];
+            const double rm = 1.0 / (be - al * cp[k - 1]);
+            cp[k] = ga * rm;
+            dp[k] = {(f.x - al * dp[k - 1].x) * rm, (f.y - al * dp[k - 1].y) * rm};
         } else if (k == nz - 1) {
             const int ki = k - 1;
             const double at = cf[3 * nz + 3], bt = cf[3 * nz + 4], ct = cf[3 * nz + 5];
             const double al = cf[3 * ki], be = cf[3 * ki + 1] + hs, ga = cf[3 * ki + 2];
-            const double2 f = rhs_at(ki);
+            const double2 f = f_top;
             double2 g = {0.0, 0.0};
             if (a.top) g = b2;
             const double lo = bt - ct * be / al, di = at - ct * ga / al;
             const double2 rr = {g.x - ct * f.x / al, g.y - ct * f.y / al};
-            const double m = di - lo * cp[k - 1];
+            const double rm = 1.0 / (di - lo * cp[k - 1]);
             cp[k] = 0.0;
-            dp[k] = {(rr.x - lo * dp[k - 1].x) / m, (rr.y - lo * dp[k - 1].y) / m};
+            dp[k] = {(rr.x - lo * dp[k - 1].x) * rm, (rr.y - lo * dp[k - 1].y) * rm};
         }
     }
     double2 x = {0.0, 0.0};
