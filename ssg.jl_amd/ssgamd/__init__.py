@@ -497,7 +497,7 @@ def get_phi3d(domain: Domain) -> np.ndarray:
 
 
 def set_coupling(domain: Domain, *, eps: float | None = 0.1, tol: float = 1e-10, maxiter: int = 50,
-                 variant: int = MA_BODY, warm: int = True) -> None:
+                 variant: int = MA_BODY, warm: int = 2) -> None:
     """Coupled stepping: re-solve the Monge-Ampère problem at every RK stage
     (``eps=None`` restores the reference's frozen φₛ).  ``warm``: 0 cold
     starts, 1 starts each solve from the last D(Φ), 2 from the linear
