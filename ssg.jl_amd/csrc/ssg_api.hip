@@ -106,6 +106,13 @@ int run_steps_coupled(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, i
     const double dt = p->dt;
     ssg_ma_diag d{};
     int rc;
+    // solves per step: the extrapolated warm start takes each stage's guess
+    // from the same stage of the previous steps
+    struct PeriodScope {
+        ssg_ctx* c;
+        ~PeriodScope() { c->ma_period = 0; }
+    } period_scope{c};
+    c->ma_period = p->scheme == SSG_RK3 ? 3 : 1;
     auto stage_tendency = [&](const double* bsrc) -> int {
         int r = ma_solve_dev(c, bsrc, &c->ma_par, &d);
         if (r) return r;
@@ -491,6 +498,7 @@ static int field_io(ssg_ctx* c, int32_t field, const double* src, double* dst, i
 
 int ssg_set_field(ssg_ctx* c, int32_t field, const double* host, int64_t n) {
     if (!host) return set_err(c, SSG_ERR_ARG, "null pointer");
+    if (c && field == SSG_FIELD_B) ma_history_reset(c);  // b no longer continues the solves' sequence
     return field_io(c, field, host, nullptr, n, false);
 }
 int ssg_get_field(ssg_ctx* c, int32_t field, double* host, int64_t n) {
@@ -499,6 +507,7 @@ int ssg_get_field(ssg_ctx* c, int32_t field, double* host, int64_t n) {
 }
 int ssg_set_field_device(ssg_ctx* c, int32_t field, const double* dev, int64_t n) {
     if (!dev) return set_err(c, SSG_ERR_ARG, "null pointer");
+    if (c && field == SSG_FIELD_B) ma_history_reset(c);
     return field_io(c, field, dev, nullptr, n, true);
 }
 int ssg_get_field_device(ssg_ctx* c, int32_t field, double* dev, int64_t n) {
@@ -634,6 +643,7 @@ int ssg_load_state(ssg_ctx* c, const void* buf, size_t bytes, ssg_time_state* s)
     if (bytes < sizeof(h)) return set_err(c, SSG_ERR_ARG, "state buffer too small");
     memcpy(&h, buf, sizeof(h));
     if (memcmp(h.magic, "SSGSTAT1", 8) != 0) return set_err(c, SSG_ERR_ARG, "not a saved step state");
+    ma_history_reset(c);
     if (h.nx != c->nx || h.ny != c->ny || h.precision != c->precision)
         return set_err(c, SSG_ERR_ARG, "state of a %dx%d grid (precision %d) does not fit this %dx%d context (%d)",
                        h.nx, h.ny, h.precision, c->nx, c->ny, c->precision);

@@ -80,8 +80,16 @@ struct ssg_ctx {
     int ma_partial_cap = 0;
     int ma_partial_used = 0;
     bool ma_have_d = false;
-    cpx<double>* ma_dprev2 = nullptr; // D̂ of the solve before the last (extrapolated warm starts)
-    int ma_hist = 0;                  // consecutive solves whose D̂ is held (0, 1, 2)
+    // D̂ of the last converged solves, for extrapolated warm starts
+    // (ma_ring[0] newest).  While the ring is non-empty ma_ring[0] holds the
+    // last D̂ and ma_dprev is scratch; otherwise ma_dprev holds it.  Buffers
+    // rotate, they are never copied; ma_spare keeps released ones.
+    cpx<double>* ma_ring[9] = {};
+    int ma_ring_n = 0;
+    cpx<double>* ma_spare[10] = {};
+    int ma_nspare = 0;
+    int ma_period = 0;                // solves per step of the running coupled stepper (RK3 3, AB2 1), else 0
+    int ma_ring_period = -1;          // the period under which the ring's entries were pushed
     double ma_last_r0 = 1e300;        // first relative residual of the last solve
     // FP32 phase of the mixed-precision solve: Φ̂ and D̂ in FP32; the three
     // derivative planes alias ma_p (an FP64 plane holds two FP32 planes)
@@ -515,6 +523,7 @@ int ma_solve_dev(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_di
 int rfft2_device(ssg_ctx* c, const double* src, cpx<double>* dst = nullptr);
 int ma_surface_outputs(ssg_ctx* c);
 void ma_free(ssg_ctx* c);
+void ma_history_reset(ssg_ctx* c);
 void adjc_free(ssg_ctx* c);
 int adj_alloc(ssg_ctx* c);
 

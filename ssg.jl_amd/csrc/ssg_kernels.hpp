@@ -928,11 +928,19 @@ static __global__ __launch_bounds__(256) void k_wmax(const double2* __restrict__
     }
 }
 
-// warm-start extrapolation over equally spaced solves: older <- 2 last - older
-static __global__ void k_extrap(const double2* __restrict__ last, double2* __restrict__ older, long n) {
+// warm-start extrapolation: out = ca a + cb b + cc c (c may be null)
+static __global__ void k_extrap(double2* __restrict__ out, const double2* __restrict__ a, double ca,
+                                const double2* __restrict__ b, double cb, const double2* __restrict__ c, double cc,
+                                long n) {
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const double2 a = last[q], b = older[q];
-        older[q] = double2{2.0 * a.x - b.x, 2.0 * a.y - b.y};
+        const double2 x = a[q], y = b[q];
+        double2 r = {ca * x.x + cb * y.x, ca * x.y + cb * y.y};
+        if (c) {
+            const double2 z = c[q];
+            r.x += cc * z.x;
+            r.y += cc * z.y;
+        }
+        out[q] = r;
     }
 }
 

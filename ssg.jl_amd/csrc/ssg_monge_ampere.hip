@@ -65,11 +65,103 @@ void ma_free(ssg_ctx* c) {
     (void)hipFree(c->ma_b2);
     (void)hipFree(c->ma32_phi);
     (void)hipFree(c->ma32_d);
-    (void)hipFree(c->ma_dprev2);
+    for (int i = 0; i < c->ma_ring_n; ++i) (void)hipFree(c->ma_ring[i]);
+    for (int i = 0; i < c->ma_nspare; ++i) (void)hipFree(c->ma_spare[i]);
+    c->ma_ring_n = c->ma_nspare = 0;
     (void)hipFree(c->ma_coef);
     (void)hipFree(c->ma_partial);
     if (c->ma_partial_host) (void)hipHostFree(c->ma_partial_host);
     (void)hipFree(c->bst);
+}
+
+// ---------------------------------------------------------------------------
+// Solve history for extrapolated warm starts.  The ring holds the D̂ of the
+// last converged solves, newest first; the buffers rotate between the ring,
+// the working buffer ma_dprev and the spares, so no D̂ is ever copied.
+// ---------------------------------------------------------------------------
+int ma_ring_cap(const ssg_ctx* c) {
+    return c->ma_period == 3 ? 9 : 3;
+}
+
+// Drop the history when the working buffer itself holds the newest D̂.
+void ma_history_drop(ssg_ctx* c) {
+    for (int i = 0; i < c->ma_ring_n; ++i) {
+        c->ma_spare[c->ma_nspare++] = c->ma_ring[i];
+        c->ma_ring[i] = nullptr;
+    }
+    c->ma_ring_n = 0;
+}
+
+// Drop the history, keeping the last D̂ as the working buffer (the plain warm
+// start's invariant).  Pointer moves only: no device work.
+void ma_history_reset(ssg_ctx* c) {
+    if (c->ma_ring_n == 0) return;
+    c->ma_spare[c->ma_nspare++] = c->ma_dprev;
+    c->ma_dprev = c->ma_ring[0];
+    for (int i = 1; i < c->ma_ring_n; ++i) c->ma_spare[c->ma_nspare++] = c->ma_ring[i];
+    for (int i = 0; i < c->ma_ring_n; ++i) c->ma_ring[i] = nullptr;
+    c->ma_ring_n = 0;
+}
+
+// The working buffer (holding the newest D̂) becomes ring[0]; the working
+// buffer is replaced by the evicted oldest entry, a spare or a new buffer.
+// Entries pushed under another period are dropped first, so the ring only
+// holds consecutive solves of one stepping pattern (their lags then mean the
+// same stage of earlier steps).
+int ma_history_push(ssg_ctx* c) {
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    if (c->ma_ring_period != c->ma_period) {
+        ma_history_drop(c);
+        c->ma_ring_period = c->ma_period;
+    }
+    const int cap = ma_ring_cap(c);
+    while (c->ma_ring_n > cap) {
+        c->ma_spare[c->ma_nspare++] = c->ma_ring[--c->ma_ring_n];
+        c->ma_ring[c->ma_ring_n] = nullptr;
+    }
+    cpx<double>* freed = nullptr;
+    if (c->ma_ring_n == cap) freed = c->ma_ring[--c->ma_ring_n];
+    for (int i = c->ma_ring_n; i > 0; --i) c->ma_ring[i] = c->ma_ring[i - 1];
+    c->ma_ring[0] = c->ma_dprev;
+    c->ma_ring_n++;
+    if (!freed && c->ma_nspare > 0) freed = c->ma_spare[--c->ma_nspare];
+    if (!freed && hipMalloc(&freed, c3 * sizeof(cpx<double>)) != hipSuccess) {
+        // no memory for more history: undo the push and keep the plain warm
+        // start (the working buffer keeps the newest D̂)
+        (void)hipGetLastError();
+        c->ma_dprev = c->ma_ring[0];
+        for (int i = 0; i + 1 < c->ma_ring_n; ++i) c->ma_ring[i] = c->ma_ring[i + 1];
+        c->ma_ring[--c->ma_ring_n] = nullptr;
+        ma_history_drop(c);
+        return SSG_OK;
+    }
+    c->ma_dprev = freed;
+    return SSG_OK;
+}
+
+// Extrapolated first D̂ into the working buffer.  With the coupled stepper's
+// period P (solves per step) and three steps of history, the guess for this
+// stage is the quadratic extrapolation over the same stage of the last three
+// steps, 3 D̂[-P] - 3 D̂[-2P] + D̂[-3P]: its error is O(dt³), and it carries no
+// error from the RK stage states' own O(dt²) offsets from the trajectory.
+// With less history: the linear 2 D̂[-1] - D̂[-2], else D̂[-1] (n >= 1).
+int ma_extrapolate(ssg_ctx* c) {
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    const int n = c->ma_ring_n, P = c->ma_period;
+    cpx<double>* const* r = c->ma_ring;
+    if (P > 0 && n >= 3 * P)
+        hipLaunchKernelGGL(k_extrap, dim3(grid1d((long)c3)), dim3(256), 0, c->stream, c->ma_dprev, r[P - 1], 3.0,
+                           r[2 * P - 1], -3.0, r[3 * P - 1], 1.0, (long)c3);
+    else if (n >= 2)
+        hipLaunchKernelGGL(k_extrap, dim3(grid1d((long)c3)), dim3(256), 0, c->stream, c->ma_dprev, r[0], 2.0, r[1],
+                           -1.0, (const double2*)nullptr, 0.0, (long)c3);
+    else {
+        HIP_OK(c, hipMemcpyAsync(c->ma_dprev, r[0], c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice, c->stream));
+        return SSG_OK;
+    }
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    return SSG_OK;
 }
 
 // Φ̂ = L⁻¹(b̂ + ε D̂) per wavenumber (D̂ term omitted when use_d = 0)
@@ -277,23 +369,14 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
     if (rc0 == SSG_OK && c->precision == SSG_FP32_MIXED) rc0 = ma_tune<float>(c);
     if (rc0) return rc0;
     const bool warm = p->warm && c->ma_have_d;
-    if (warm && p->warm >= 2) {
-        // extrapolated warm start: with two held solves the first iterate
-        // starts from 2 D̂_k - D̂_{k-1}; the residual stays exact, since the
-        // iterate is L⁻¹ of whatever D̂ it was built from.  The guess is
-        // written over D̂_{k-1} and the two buffers swap roles, so the solve
-        // iterates in that buffer while D̂_k stays held for the next solve.
-        if (!c->ma_dprev2) HIP_OK(c, hipMalloc(&c->ma_dprev2, c3 * sizeof(cpx<double>)));
-        if (c->ma_hist >= 2) {
-            hipLaunchKernelGGL(k_extrap, dim3(grid1d((long)c3)), dim3(256), 0, c->stream, c->ma_dprev, c->ma_dprev2,
-                               (long)c3);
-            c->launches++;
-            HIP_OK(c, hipGetLastError());
-            std::swap(c->ma_dprev, c->ma_dprev2);
-        } else {
-            HIP_OK(c, hipMemcpyAsync(c->ma_dprev2, c->ma_dprev, c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice,
-                                     c->stream));
-        }
+    const bool extrap = warm && p->warm >= 2;
+    if (!extrap) {
+        ma_history_reset(c);  // the last D̂ back in the working buffer
+    } else {
+        // extrapolated warm start (with an empty ring the working buffer
+        // holds the last D̂, a plain warm start); the residual stays exact,
+        // since the first iterate is L⁻¹ of whatever D̂ it was built from
+        if (c->ma_ring_n > 0 && (rc0 = ma_extrapolate(c))) return rc0;
     }
     memset(d, 0, sizeof(*d));
     int it = 0, rc;
@@ -344,8 +427,8 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
         if ((rc = cpx_convert<float, double>(c, c->ma32_d, c->ma_dprev, c3))) return rc;
         if ((rc = ma_tridiag<float, double>(c, p, 1, c->ma32_d, c->ma_phi))) return rc;
         if (stop && (!std::isfinite(rr) || rr > 100.0 * best)) {
+            ma_history_drop(c);
             c->ma_have_d = false;
-            c->ma_hist = 0;
             c->ma_last_r0 = 1e300;
             return SSG_OK;
         }
@@ -359,9 +442,17 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
         if (record(true)) break;
         if ((rc = ma_tridiag<double, double>(c, p, 1, c->ma_dprev, c->ma_phi))) return rc;
     }
+    if (!std::isfinite(rr) || rr > 100.0 * best) {  // diverged: keep no warm start
+        ma_history_drop(c);
+        c->ma_have_d = false;
+        c->ma_last_r0 = 1e300;
+        return SSG_OK;
+    }
+    // the working buffer holds the final iterate's D̂
     c->ma_have_d = true;
-    c->ma_hist = (warm && p->warm >= 2) ? 2 : 1;
     c->ma_last_r0 = d->history[0];
+    if (p->warm >= 2 && d->converged) return ma_history_push(c);
+    ma_history_drop(c);
     return SSG_OK;
 }
 
@@ -418,6 +509,12 @@ int ma_solve_dev(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_di
     if (rc) return rc;
     if ((rc = rfft2_device(c, bs, c->ma_b2))) return rc;
     if ((rc = ma_iterate(c, p, d))) return rc;
+    if (!d->converged && p->warm) {
+        // a warm start that did not lead to convergence is retried cold
+        ssg_ma_params cold = *p;
+        cold.warm = 0;
+        if ((rc = ma_iterate(c, &cold, d))) return rc;
+    }
     if ((rc = ma_surface_outputs(c))) return rc;
     c->last_diag = *d;
     return SSG_OK;
@@ -1089,6 +1186,7 @@ int ssg_get_phi3d(ssg_ctx* c, double* host, int64_t n) {
 
 int ssg_set_coupling(ssg_ctx* c, const ssg_ma_params* p) {
     if (!c) return SSG_ERR_ARG;
+    ma_history_reset(c);
     if (!p) {
         c->coupled = false;
         return SSG_OK;
@@ -1146,6 +1244,7 @@ int ssg_adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_param
     const size_t bytes = (size_t)c->nx * c->ny * sizeof(double);
     HIP_OK(c, hipMemcpyAsync(c->adj_lam, lam_in, bytes, hipMemcpyHostToDevice, c->stream));
     if (nsteps > 0 && (rc = adjoint_coupled(c, p, mp, step_start, nsteps, checkpoint_every))) return rc;
+    ma_history_reset(c);  // the recomputation's solves do not continue a forward sequence
     HIP_OK(c, hipMemcpyAsync(lam_out, c->adj_lam, bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     return SSG_OK;
