@@ -107,9 +107,10 @@ typedef struct {
     double tol;        /* relative residual target ‖r‖₂ / ‖b + εD‖₂ on interior levels */
     int32_t maxiter;   /* src/poisson.jl:874 default 50 */
     int32_t variant;   /* SSG_MA_BODY or SSG_MA_SURFACE */
-    int32_t warm;      /* 0: cold start; 1: start from the previous solve's D(Φ); 2: from the linear
-                          extrapolation 2 D̂_k - D̂_{k-1} of the last two solves (for equally spaced
-                          solves: the RK3 stages, abscissae 0, 1/3, 2/3, and AB2 steps) */
+    int32_t warm;      /* 0: cold start; 1: start from the previous solve's D(Φ); 2: extrapolate from the
+                          last converged solves: in coupled stepping quadratically over the same stage of
+                          the last three steps, otherwise 2 D̂_k - D̂_{k-1}.  Writing b or changing the
+                          coupling resets that history; a failed warm start is retried cold. */
 } ssg_ma_params;
 
 /* solve_ssg_equation diagnostics (src/poisson.jl:928-958) */
