@@ -197,7 +197,7 @@ def measure_adjoint(ssg, steps=100, every=10):
     # coupled
     f.bs = b0
     ssg.solve_monge_ampere(f, d, eps=eps, tol=1e-10, maxiter=100, variant=ssg.MA_BODY)
-    ssg.set_coupling(d, eps=eps, tol=1e-10, maxiter=100, variant=ssg.MA_BODY, warm=True)
+    ssg.set_coupling(d, eps=eps, tol=1e-10, maxiter=100, variant=ssg.MA_BODY, warm=2)
     st = ssg.TimeState(0.0, f)
     t0 = time.perf_counter()
     ssg.timestep(f, d, params, st, steps)
@@ -206,7 +206,7 @@ def measure_adjoint(ssg, steps=100, every=10):
     ssg.set_coupling(d, eps=None)
     f.bs = b0
     g = ssg.timestep_adjoint_coupled(d, params, 0, steps, (bT - b_tgt) * dA, eps=eps, tol=1e-10, maxiter=100,
-                                     variant=ssg.MA_BODY, warm=True, checkpoint_every=every)
+                                     variant=ssg.MA_BODY, warm=2, checkpoint_every=every)
     t2 = time.perf_counter()
     d.close()
     out["coupled_forward_s"] = t1 - t0

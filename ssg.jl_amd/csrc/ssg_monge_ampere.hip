@@ -1032,6 +1032,9 @@ int adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* m
         return set_err(c, SSG_ERR_HIP, "checkpoint allocation failed");
     }
     const bool had_history = c->ab2_have_history;
+    // solves per step, for the extrapolated warm start's same-stage history
+    const int saved_period = c->ma_period;
+    c->ma_period = ab2 ? 1 : 3;
     auto step = [&](int64_t step_after) -> int {
         return ab2 ? coupled_step_record_ab2(c, p, mp, step_after) : coupled_step_record(c, p, mp, step_after);
     };
@@ -1061,6 +1064,12 @@ int adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* m
                 HIP_OK(c, hipGetLastError());
             }
         }
+        // The solves below run in three unbroken sequences: the forward pass,
+        // each segment's recomputation (forward from its checkpoint), and
+        // each segment's sweep (backward, one step at a time).  The
+        // extrapolated warm start's history is reset at every break, so it
+        // only ever extrapolates along one sequence, in its own direction.
+        ma_history_reset(c);
         for (int64_t j = 0; j < nsteps; ++j) {
             if (j % every == 0 && (rc = save_ck(j / every))) return rc;
             if ((rc = step(step0 + j + 1))) return rc;
@@ -1070,6 +1079,7 @@ int adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* m
             const int64_t len = std::min<int64_t>(every, nsteps - j0);
             // recompute the segment's per-step states from its checkpoint
             if ((rc = load_ck(sg))) return rc;
+            ma_history_reset(c);
             for (int64_t k = 0; k < len; ++k) {
                 HIP_OK(c, hipMemcpyAsync(seg + (size_t)k * n2, c->b, bytes, hipMemcpyDeviceToDevice, c->stream));
                 if (k + 1 < len && (rc = step(step0 + j0 + k + 1))) return rc;
@@ -1078,6 +1088,7 @@ int adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* m
                 HIP_OK(c, hipMemcpyAsync(c->b, seg + (size_t)jj * n2, bytes, hipMemcpyDeviceToDevice, c->stream));
                 return SSG_OK;
             };
+            ma_history_reset(c);
             rc = ab2 ? adjoint_coupled_sweep_ab2(c, p, mp, step0 + j0, len, state_at)
                      : adjoint_coupled_sweep(c, p, mp, step0 + j0, len, state_at);
             if (rc) return rc;
@@ -1087,6 +1098,7 @@ int adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* m
     };
     const int rc = nsteps > 0 ? body() : SSG_OK;
     (void)hipStreamSynchronize(c->stream);
+    c->ma_period = saved_period;
     c->ab2_have_history = had_history;
     (void)hipFree(ck);
     (void)hipFree(seg);

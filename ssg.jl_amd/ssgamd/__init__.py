@@ -615,7 +615,7 @@ def ma_adjoint(domain: Domain, phibar, *, eps: float = 0.1, tol: float = 1e-10, 
 
 def timestep_adjoint_coupled(domain: Domain, params: TimeParams, step_start: int, nsteps: int, lam, *,
                              eps: float = 0.1, tol: float = 1e-10, maxiter: int = 50, variant: int = MA_BODY,
-                             warm: bool = True, checkpoint_every: int = 10) -> np.ndarray:
+                             warm: int = 2, checkpoint_every: int = 10) -> np.ndarray:
     """Vector-Jacobian product of ``nsteps`` coupled RK3 steps (Monge-Ampère
     solve per stage) from the domain's current ``bs`` at step ``step_start``:
     returns (∂b_end/∂b_start)ᵀ lam.  Checkpoints every ``checkpoint_every``

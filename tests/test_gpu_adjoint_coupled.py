@@ -67,16 +67,16 @@ def test_ma_adjoint_against_fd(gpu, variant):
     d.close()
 
 
-def coupled_forward(gpu, d, f, b0, params, nsteps, eps, variant):
+def coupled_forward(gpu, d, f, b0, params, nsteps, eps, variant, warm=1):
     f.bs = b0
-    gpu.set_coupling(d, eps=eps, tol=1e-13, maxiter=200, variant=variant, warm=True)
+    gpu.set_coupling(d, eps=eps, tol=1e-13, maxiter=200, variant=variant, warm=warm)
     st = gpu.TimeState(0.0, f)
     gpu.timestep(f, d, params, st, nsteps)
     gpu.set_coupling(d, eps=None)
     return f.bs
 
 
-def coupled_gradient_check(gpu, n, nz, nsteps, dt, every, ndir, filter_freq, seed_dirs, scheme=1):
+def coupled_gradient_check(gpu, n, nz, nsteps, dt, every, ndir, filter_freq, seed_dirs, scheme=1, warm=1):
     eps = 0.1
     b0 = smooth(n, 0, k0=6.0)
     b_tgt = smooth(n, 1, k0=6.0)
@@ -85,10 +85,10 @@ def coupled_gradient_check(gpu, n, nz, nsteps, dt, every, ndir, filter_freq, see
                             filter_cutoff=0.65)
     d = gpu.make_domain(n, n, nz)
     f = gpu.allocate_fields(d)
-    bT = coupled_forward(gpu, d, f, b0, params, nsteps, eps, gpu.MA_BODY)
+    bT = coupled_forward(gpu, d, f, b0, params, nsteps, eps, gpu.MA_BODY, warm)
     f.bs = b0
     grad = gpu.timestep_adjoint_coupled(d, params, 0, nsteps, (bT - b_tgt) * dA, eps=eps, tol=1e-13, maxiter=200,
-                                        variant=gpu.MA_BODY, warm=True, checkpoint_every=every)
+                                        variant=gpu.MA_BODY, warm=warm, checkpoint_every=every)
     assert np.abs(f.bs - b0).max() == 0.0   # the context is left at the start state
     rng = np.random.default_rng(seed_dirs)
     h = 1e-6
@@ -96,8 +96,8 @@ def coupled_gradient_check(gpu, n, nz, nsteps, dt, every, ndir, filter_freq, see
     for k in range(ndir):
         v = rng.standard_normal((n, n))
         v /= np.sqrt(np.mean(v * v))
-        rp = coupled_forward(gpu, d, f, b0 + h * v, params, nsteps, eps, gpu.MA_BODY) - b_tgt
-        rm = coupled_forward(gpu, d, f, b0 - h * v, params, nsteps, eps, gpu.MA_BODY) - b_tgt
+        rp = coupled_forward(gpu, d, f, b0 + h * v, params, nsteps, eps, gpu.MA_BODY, warm) - b_tgt
+        rm = coupled_forward(gpu, d, f, b0 - h * v, params, nsteps, eps, gpu.MA_BODY, warm) - b_tgt
         fd = 0.5 * np.sum((rp - rm) * (rp + rm)) * dA / (2 * h)
         an = np.sum(grad * v)
         rel = abs(fd - an) / abs(an)
@@ -115,8 +115,9 @@ def test_coupled_adjoint_small_with_segments(gpu):
 
 def test_coupled_adjoint_ab2_with_segments(gpu):
     """Coupled AB2 (forward Euler first step), 64²×6, 7 steps, checkpoints
-    every 3 carrying the held tendency, filter every 2."""
-    worst = coupled_gradient_check(gpu, 64, 6, 7, 2e-3, 3, 3, 2, 6, scheme=0)
+    every 3 carrying the held tendency, filter every 2; extrapolated warm
+    starts in the forward runs and the adjoint's recomputation."""
+    worst = coupled_gradient_check(gpu, 64, 6, 7, 2e-3, 3, 3, 2, 6, scheme=0, warm=2)
     assert worst <= 1e-6
 
 
@@ -124,8 +125,9 @@ def test_coupled_adjoint_configuration(gpu):
     """The adjoint configuration with the Monge-Ampère solve in the step:
     256²×16, ε = 0.1, RK3 dt = 1e-3, 100 steps, checkpoint every 10 steps
     with recompute; central differences along 8 random unit-RMS directions,
-    step 1e-6, FP64."""
-    worst = coupled_gradient_check(gpu, 256, 16, 100, 1e-3, 10, 8, 10, 2)
+    step 1e-6, FP64; extrapolated warm starts (same RK stage of the last three
+    steps) in the forward runs and along each of the adjoint's sequences."""
+    worst = coupled_gradient_check(gpu, 256, 16, 100, 1e-3, 10, 8, 10, 2, warm=2)
     assert worst <= 1e-6
 
 
@@ -146,10 +148,10 @@ def test_coupled_adjoint_against_cpu_forward_mode(gpu):
                             filter_cutoff=0.65)
     d = gpu.make_domain(n, n, nz)
     f = gpu.allocate_fields(d)
-    bT = coupled_forward(gpu, d, f, b0, params, nsteps, eps, gpu.MA_BODY)
+    bT = coupled_forward(gpu, d, f, b0, params, nsteps, eps, gpu.MA_BODY, 2)
     f.bs = b0
     grad = gpu.timestep_adjoint_coupled(d, params, 0, nsteps, (bT - b_tgt) * dA, eps=eps, tol=1e-13, maxiter=200,
-                                        variant=gpu.MA_BODY, warm=True, checkpoint_every=3)
+                                        variant=gpu.MA_BODY, warm=2, checkpoint_every=3)
     d.close()
     rng = np.random.default_rng(11)
     dirs = []
