@@ -44,6 +44,19 @@ struct PathBuffers {
 
 using namespace ssg_impl;
 
+// The iterates of the last converged solves of one fixed-point iteration, for
+// extrapolated warm starts (ring[0] newest).  While the ring is non-empty
+// ring[0] holds the last iterate and the solver's working buffer is scratch;
+// otherwise the working buffer holds it.  Buffers rotate between the ring,
+// the working buffer and the spares; no iterate is ever copied.
+struct SolveHistory {
+    cpx<double>* ring[9] = {};
+    int n = 0;
+    cpx<double>* spare[10] = {};
+    int nspare = 0;
+    int period = -1;  // the solves-per-step pattern under which the entries were pushed
+};
+
 struct ssg_ctx {
     int nx = 0, ny = 0, nz = 0;
     double lx = 0, ly = 0, lz = 0;
@@ -80,16 +93,8 @@ struct ssg_ctx {
     int ma_partial_cap = 0;
     int ma_partial_used = 0;
     bool ma_have_d = false;
-    // D̂ of the last converged solves, for extrapolated warm starts
-    // (ma_ring[0] newest).  While the ring is non-empty ma_ring[0] holds the
-    // last D̂ and ma_dprev is scratch; otherwise ma_dprev holds it.  Buffers
-    // rotate, they are never copied; ma_spare keeps released ones.
-    cpx<double>* ma_ring[9] = {};
-    int ma_ring_n = 0;
-    cpx<double>* ma_spare[10] = {};
-    int ma_nspare = 0;
+    SolveHistory ma_hist;             // D̂ of the last converged solves (working buffer: ma_dprev)
     int ma_period = 0;                // solves per step of the running coupled stepper (RK3 3, AB2 1), else 0
-    int ma_ring_period = -1;          // the period under which the ring's entries were pushed
     double ma_last_r0 = 1e300;        // first relative residual of the last solve
     // FP32 phase of the mixed-precision solve: Φ̂ and D̂ in FP32; the three
     // derivative planes alias ma_p (an FP64 plane holds two FP32 planes)
@@ -120,6 +125,8 @@ struct ssg_ctx {
     double* adjc_bst[3] = {nullptr, nullptr, nullptr};       // stage buoyancies of the step being reversed
     cpx<double>* adjc_phis[3] = {nullptr, nullptr, nullptr}; // their Monge-Ampère solutions
     ssg_ma_diag adjc_diag{};
+    SolveHistory adjc_hist;           // q of the last transposed solves (working buffer: adjc_q)
+    int adjc_period = 0;              // transposed solves per step of the running adjoint sweep, else 0
     struct GraphCache {
         hipGraphExec_t exec = nullptr;
         int len = 0;

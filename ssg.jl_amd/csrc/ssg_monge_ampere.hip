@@ -4,6 +4,8 @@
 
 namespace ssg_impl {
 
+void hist_free(SolveHistory& h);
+
 // ---------------------------------------------------------------------------
 // Monge-Ampère solve (FP64)
 // ---------------------------------------------------------------------------
@@ -65,9 +67,7 @@ void ma_free(ssg_ctx* c) {
     (void)hipFree(c->ma_b2);
     (void)hipFree(c->ma32_phi);
     (void)hipFree(c->ma32_d);
-    for (int i = 0; i < c->ma_ring_n; ++i) (void)hipFree(c->ma_ring[i]);
-    for (int i = 0; i < c->ma_nspare; ++i) (void)hipFree(c->ma_spare[i]);
-    c->ma_ring_n = c->ma_nspare = 0;
+    hist_free(c->ma_hist);
     (void)hipFree(c->ma_coef);
     (void)hipFree(c->ma_partial);
     if (c->ma_partial_host) (void)hipHostFree(c->ma_partial_host);
@@ -75,93 +75,104 @@ void ma_free(ssg_ctx* c) {
 }
 
 // ---------------------------------------------------------------------------
-// Solve history for extrapolated warm starts.  The ring holds the D̂ of the
-// last converged solves, newest first; the buffers rotate between the ring,
-// the working buffer ma_dprev and the spares, so no D̂ is ever copied.
+// Solve history for extrapolated warm starts (struct SolveHistory).  The
+// ring holds the iterates of the last converged solves, newest first; the
+// buffers rotate between the ring, the solver's working buffer and the
+// spares, so no iterate is ever copied.
 // ---------------------------------------------------------------------------
-int ma_ring_cap(const ssg_ctx* c) {
-    return c->ma_period == 3 ? 9 : 3;
+int hist_cap(int period) {
+    return period == 3 ? 9 : 3;
 }
 
-// Drop the history when the working buffer itself holds the newest D̂.
-void ma_history_drop(ssg_ctx* c) {
-    for (int i = 0; i < c->ma_ring_n; ++i) {
-        c->ma_spare[c->ma_nspare++] = c->ma_ring[i];
-        c->ma_ring[i] = nullptr;
+void hist_free(SolveHistory& h) {
+    for (int i = 0; i < h.n; ++i) (void)hipFree(h.ring[i]);
+    for (int i = 0; i < h.nspare; ++i) (void)hipFree(h.spare[i]);
+    h = SolveHistory{};
+}
+
+// Drop the history when the working buffer itself holds the newest iterate.
+void hist_drop(SolveHistory& h) {
+    for (int i = 0; i < h.n; ++i) {
+        h.spare[h.nspare++] = h.ring[i];
+        h.ring[i] = nullptr;
     }
-    c->ma_ring_n = 0;
+    h.n = 0;
 }
 
-// Drop the history, keeping the last D̂ as the working buffer (the plain warm
-// start's invariant).  Pointer moves only: no device work.
-void ma_history_reset(ssg_ctx* c) {
-    if (c->ma_ring_n == 0) return;
-    c->ma_spare[c->ma_nspare++] = c->ma_dprev;
-    c->ma_dprev = c->ma_ring[0];
-    for (int i = 1; i < c->ma_ring_n; ++i) c->ma_spare[c->ma_nspare++] = c->ma_ring[i];
-    for (int i = 0; i < c->ma_ring_n; ++i) c->ma_ring[i] = nullptr;
-    c->ma_ring_n = 0;
+// Drop the history, keeping the last iterate as the working buffer (the
+// plain warm start's invariant).  Pointer moves only: no device work.
+void hist_reset(SolveHistory& h, cpx<double>*& work) {
+    if (h.n == 0) return;
+    h.spare[h.nspare++] = work;
+    work = h.ring[0];
+    for (int i = 1; i < h.n; ++i) h.spare[h.nspare++] = h.ring[i];
+    for (int i = 0; i < h.n; ++i) h.ring[i] = nullptr;
+    h.n = 0;
 }
 
-// The working buffer (holding the newest D̂) becomes ring[0]; the working
-// buffer is replaced by the evicted oldest entry, a spare or a new buffer.
-// Entries pushed under another period are dropped first, so the ring only
-// holds consecutive solves of one stepping pattern (their lags then mean the
-// same stage of earlier steps).
-int ma_history_push(ssg_ctx* c) {
+// The working buffer (holding the newest iterate) becomes ring[0]; the
+// working buffer is replaced by the evicted oldest entry, a spare or a new
+// buffer.  Entries pushed under another period are dropped first, so the
+// ring only holds consecutive solves of one pattern (their lags then mean
+// the same stage of earlier steps).
+int hist_push(ssg_ctx* c, SolveHistory& h, cpx<double>*& work, int period) {
     const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
-    if (c->ma_ring_period != c->ma_period) {
-        ma_history_drop(c);
-        c->ma_ring_period = c->ma_period;
+    if (h.period != period) {
+        hist_drop(h);
+        h.period = period;
     }
-    const int cap = ma_ring_cap(c);
-    while (c->ma_ring_n > cap) {
-        c->ma_spare[c->ma_nspare++] = c->ma_ring[--c->ma_ring_n];
-        c->ma_ring[c->ma_ring_n] = nullptr;
+    const int cap = hist_cap(period);
+    while (h.n > cap) {
+        h.spare[h.nspare++] = h.ring[--h.n];
+        h.ring[h.n] = nullptr;
     }
     cpx<double>* freed = nullptr;
-    if (c->ma_ring_n == cap) freed = c->ma_ring[--c->ma_ring_n];
-    for (int i = c->ma_ring_n; i > 0; --i) c->ma_ring[i] = c->ma_ring[i - 1];
-    c->ma_ring[0] = c->ma_dprev;
-    c->ma_ring_n++;
-    if (!freed && c->ma_nspare > 0) freed = c->ma_spare[--c->ma_nspare];
+    if (h.n == cap) freed = h.ring[--h.n];
+    for (int i = h.n; i > 0; --i) h.ring[i] = h.ring[i - 1];
+    h.ring[0] = work;
+    h.n++;
+    if (!freed && h.nspare > 0) freed = h.spare[--h.nspare];
     if (!freed && hipMalloc(&freed, c3 * sizeof(cpx<double>)) != hipSuccess) {
         // no memory for more history: undo the push and keep the plain warm
-        // start (the working buffer keeps the newest D̂)
+        // start (the working buffer keeps the newest iterate)
         (void)hipGetLastError();
-        c->ma_dprev = c->ma_ring[0];
-        for (int i = 0; i + 1 < c->ma_ring_n; ++i) c->ma_ring[i] = c->ma_ring[i + 1];
-        c->ma_ring[--c->ma_ring_n] = nullptr;
-        ma_history_drop(c);
+        work = h.ring[0];
+        for (int i = 0; i + 1 < h.n; ++i) h.ring[i] = h.ring[i + 1];
+        h.ring[--h.n] = nullptr;
+        hist_drop(h);
         return SSG_OK;
     }
-    c->ma_dprev = freed;
+    work = freed;
     return SSG_OK;
 }
 
-// Extrapolated first D̂ into the working buffer.  With the coupled stepper's
-// period P (solves per step) and three steps of history, the guess for this
-// stage is the quadratic extrapolation over the same stage of the last three
-// steps, 3 D̂[-P] - 3 D̂[-2P] + D̂[-3P]: its error is O(dt³), and it carries no
+// Extrapolated first iterate into the working buffer (needs h.n >= 1).  With
+// a stepping period P (solves per step) and three steps of history, the
+// guess is the quadratic extrapolation over the same stage of the last three
+// steps, 3 x[-P] - 3 x[-2P] + x[-3P]: its error is O(dt³), and it carries no
 // error from the RK stage states' own O(dt²) offsets from the trajectory.
-// With less history: the linear 2 D̂[-1] - D̂[-2], else D̂[-1] (n >= 1).
-int ma_extrapolate(ssg_ctx* c) {
+// With less history: the linear 2 x[-1] - x[-2], else x[-1].
+int hist_extrapolate(ssg_ctx* c, const SolveHistory& h, cpx<double>* work, int period) {
     const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
-    const int n = c->ma_ring_n, P = c->ma_period;
-    cpx<double>* const* r = c->ma_ring;
+    const int n = h.n, P = period;
+    cpx<double>* const* r = h.ring;
     if (P > 0 && n >= 3 * P)
-        hipLaunchKernelGGL(k_extrap, dim3(grid1d((long)c3)), dim3(256), 0, c->stream, c->ma_dprev, r[P - 1], 3.0,
+        hipLaunchKernelGGL(k_extrap, dim3(grid1d((long)c3)), dim3(256), 0, c->stream, work, r[P - 1], 3.0,
                            r[2 * P - 1], -3.0, r[3 * P - 1], 1.0, (long)c3);
     else if (n >= 2)
-        hipLaunchKernelGGL(k_extrap, dim3(grid1d((long)c3)), dim3(256), 0, c->stream, c->ma_dprev, r[0], 2.0, r[1],
-                           -1.0, (const double2*)nullptr, 0.0, (long)c3);
+        hipLaunchKernelGGL(k_extrap, dim3(grid1d((long)c3)), dim3(256), 0, c->stream, work, r[0], 2.0, r[1], -1.0,
+                           (const double2*)nullptr, 0.0, (long)c3);
     else {
-        HIP_OK(c, hipMemcpyAsync(c->ma_dprev, r[0], c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice, c->stream));
+        HIP_OK(c, hipMemcpyAsync(work, r[0], c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice, c->stream));
         return SSG_OK;
     }
     c->launches++;
     HIP_OK(c, hipGetLastError());
     return SSG_OK;
+}
+
+void ma_history_reset(ssg_ctx* c) {
+    hist_reset(c->ma_hist, c->ma_dprev);
 }
 
 // Φ̂ = L⁻¹(b̂ + ε D̂) per wavenumber (D̂ term omitted when use_d = 0)
@@ -376,7 +387,7 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
         // extrapolated warm start (with an empty ring the working buffer
         // holds the last D̂, a plain warm start); the residual stays exact,
         // since the first iterate is L⁻¹ of whatever D̂ it was built from
-        if (c->ma_ring_n > 0 && (rc0 = ma_extrapolate(c))) return rc0;
+        if (c->ma_hist.n > 0 && (rc0 = hist_extrapolate(c, c->ma_hist, c->ma_dprev, c->ma_period))) return rc0;
     }
     memset(d, 0, sizeof(*d));
     int it = 0, rc;
@@ -427,7 +438,7 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
         if ((rc = cpx_convert<float, double>(c, c->ma32_d, c->ma_dprev, c3))) return rc;
         if ((rc = ma_tridiag<float, double>(c, p, 1, c->ma32_d, c->ma_phi))) return rc;
         if (stop && (!std::isfinite(rr) || rr > 100.0 * best)) {
-            ma_history_drop(c);
+            hist_drop(c->ma_hist);
             c->ma_have_d = false;
             c->ma_last_r0 = 1e300;
             return SSG_OK;
@@ -443,7 +454,7 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
         if ((rc = ma_tridiag<double, double>(c, p, 1, c->ma_dprev, c->ma_phi))) return rc;
     }
     if (!std::isfinite(rr) || rr > 100.0 * best) {  // diverged: keep no warm start
-        ma_history_drop(c);
+        hist_drop(c->ma_hist);
         c->ma_have_d = false;
         c->ma_last_r0 = 1e300;
         return SSG_OK;
@@ -451,8 +462,8 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
     // the working buffer holds the final iterate's D̂
     c->ma_have_d = true;
     c->ma_last_r0 = d->history[0];
-    if (p->warm >= 2 && d->converged) return ma_history_push(c);
-    ma_history_drop(c);
+    if (p->warm >= 2 && d->converged) return hist_push(c, c->ma_hist, c->ma_dprev, c->ma_period);
+    hist_drop(c->ma_hist);
     return SSG_OK;
 }
 
@@ -654,6 +665,7 @@ int adjoint_steps_ab2(ssg_ctx* c, const ssg_time_params* p, int64_t step_end, in
 // checkpoints.
 // ---------------------------------------------------------------------------
 void adjc_free(ssg_ctx* c) {
+    hist_free(c->adjc_hist);
     for (int i = 0; i < 3; ++i) {
         (void)hipFree(c->adjc_pd[i]);
         (void)hipFree(c->adjc_bst[i]);
@@ -793,15 +805,23 @@ int ma_tridiag_adj(ssg_ctx* c, const ssg_ma_params* p, int use_q) {
 
 // Transposed linearised Monge-Ampère solve at the current ma_phi (whose
 // physical derivative fields are in adjc_pd): adjc_phibar = F(φ̄) in,
-// adjc_bbar = F(b̄) out.
-int ma_adjoint_solve(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
+// adjc_bbar = F(b̄) out.  With `warm`, inside an adjoint sweep, the first
+// iterate is extrapolated from the same stage of the last steps (the sweep
+// runs backward, and so does the extrapolation); otherwise it starts at 0.
+int ma_adjoint_solve(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d, bool warm) {
     const double scale = 1.0 / ((double)c->nx * c->ny);
     const int lines = c->ncols_full * c->nz;
     const size_t c3 = (size_t)lines * c->ny;
     memset(d, 0, sizeof(*d));
-    HIP_OK(c, hipMemsetAsync(c->adjc_q, 0, c3 * sizeof(cpx<double>), c->stream));
-    int rc = ma_tridiag_adj(c, p, 0);
-    if (rc) return rc;
+    int rc;
+    const bool start = warm && c->adjc_hist.n > 0;
+    if (start) {
+        if ((rc = hist_extrapolate(c, c->adjc_hist, c->adjc_q, c->adjc_period))) return rc;
+    } else {
+        hist_reset(c->adjc_hist, c->adjc_q);
+        HIP_OK(c, hipMemsetAsync(c->adjc_q, 0, c3 * sizeof(cpx<double>), c->stream));
+    }
+    if ((rc = ma_tridiag_adj(c, p, start ? 1 : 0))) return rc;
     double best = 1e300;
     for (int it = 0;; ++it) {
         ColsArgs<double> a{};
@@ -843,9 +863,14 @@ int ma_adjoint_solve(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
         if ((rc = ma_tridiag_adj(c, p, 1))) return rc;
     }
     c->adjc_diag = *d;
-    if (!d->converged)
+    if (!d->converged) {
+        hist_drop(c->adjc_hist);
         return set_err(c, SSG_ERR_NOT_CONVERGED, "adjoint Monge-Ampere solve did not converge (rel. residual %.3e)",
                        d->relative_residual);
+    }
+    // adjc_q holds the last iterate
+    if (warm) return hist_push(c, c->adjc_hist, c->adjc_q, c->adjc_period);
+    hist_drop(c->adjc_hist);
     return SSG_OK;
 }
 
@@ -920,7 +945,7 @@ int adjc_stage(ssg_ctx* c, const ssg_ma_params* mp, const double* kbar, int st) 
     if ((rc = launch_cols_args<double, OP_PHIBAR>(c, ph, c->ncols_full, 1))) return rc;
     // through the linearised Monge-Ampère solve
     ssg_ma_diag d{};
-    if ((rc = ma_adjoint_solve(c, mp, &d))) return rc;
+    if ((rc = ma_adjoint_solve(c, mp, &d, mp->warm >= 2 && c->adjc_period > 0))) return rc;
     if ((rc = irfft2_device(c, c->adjc_bbar, c->adjc_tmp))) return rc;
     return lincomb(c, c->adjc_g, 1.0, c->adjc_g, 1.0, c->adjc_tmp);
 }
@@ -1032,9 +1057,12 @@ int adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* m
         return set_err(c, SSG_ERR_HIP, "checkpoint allocation failed");
     }
     const bool had_history = c->ab2_have_history;
-    // solves per step, for the extrapolated warm start's same-stage history
+    // solves per step, for the extrapolated warm starts' same-stage history;
+    // the transposed solves run as one backward sequence over all segments
     const int saved_period = c->ma_period;
     c->ma_period = ab2 ? 1 : 3;
+    c->adjc_period = ab2 ? 1 : 3;
+    hist_reset(c->adjc_hist, c->adjc_q);
     auto step = [&](int64_t step_after) -> int {
         return ab2 ? coupled_step_record_ab2(c, p, mp, step_after) : coupled_step_record(c, p, mp, step_after);
     };
@@ -1099,6 +1127,8 @@ int adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* m
     const int rc = nsteps > 0 ? body() : SSG_OK;
     (void)hipStreamSynchronize(c->stream);
     c->ma_period = saved_period;
+    c->adjc_period = 0;
+    hist_reset(c->adjc_hist, c->adjc_q);
     c->ab2_have_history = had_history;
     (void)hipFree(ck);
     (void)hipFree(seg);
@@ -1276,7 +1306,7 @@ int ssg_ma_adjoint(ssg_ctx* c, const ssg_ma_params* mp, const double* phibar, do
     if ((rc = rfft2_device(c, c->adjc_tmp, c->adjc_phibar))) return rc;
     if ((rc = ma_phys_derivs(c))) return rc;
     ssg_ma_diag d{};
-    rc = ma_adjoint_solve(c, mp, &d);
+    rc = ma_adjoint_solve(c, mp, &d, false);
     if (diag) *diag = d;
     if (rc) return rc;
     if ((rc = irfft2_device(c, c->adjc_bbar, c->adjc_tmp))) return rc;
