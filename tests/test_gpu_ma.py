@@ -338,3 +338,40 @@ def test_extrapolated_warm_start(gpu, prec):
     assert c1 and c2
     assert e < 1e-8
     assert it2 <= it1
+
+
+def test_warm_start_history_across_mode_changes(gpu):
+    """The extrapolated warm start's solve history across every change of
+    pattern: coupled RK3 (three solves per step), coupled AB2 (one), a
+    standalone solve, a new b written mid-run, the coupling switched off and
+    on.  Each pattern change must drop or reset the history; the result must
+    follow the same trajectory as the plain warm start to the solve
+    tolerance."""
+    n, nz, eps, dt = 64, 8, 0.1, 1e-3
+    b0 = smooth_b(n, 4, 6.0)
+    b1 = smooth_b(n, 5, 6.0)
+    out = {}
+    for warm in (1, 2):
+        d = gpu.make_domain(n, n, nz)
+        f = gpu.allocate_fields(d)
+        f.bs = b0
+        gpu.set_coupling(d, eps=eps, tol=1e-12, maxiter=80, variant=gpu.MA_BODY, warm=warm)
+        st = gpu.TimeState(0.0, f)
+        rk3 = gpu.TimeParams(dt, scheme=gpu.RK3, filter_freq=4, filter_strength=36.0, filter_order=8,
+                             filter_cutoff=0.65)
+        ab2 = gpu.TimeParams(dt, scheme=gpu.AB2_LowStorage, filter_freq=0)
+        gpu.timestep(f, d, rk3, st, 6)
+        gpu.timestep(f, d, ab2, gpu.TimeState(0.0, f), 5)
+        diag = gpu.solve_monge_ampere(f, d, eps=eps, tol=1e-12, maxiter=80, variant=gpu.MA_BODY, warm=warm)
+        assert diag["converged"]
+        gpu.timestep(f, d, rk3, st, 4)
+        f.bs = 0.5 * (f.bs + b1)            # a new b mid-run
+        gpu.timestep(f, d, rk3, st, 4)
+        gpu.set_coupling(d, eps=None)
+        gpu.set_coupling(d, eps=eps, tol=1e-12, maxiter=80, variant=gpu.MA_BODY, warm=warm)
+        gpu.timestep(f, d, rk3, st, 4)
+        out[warm] = f.bs
+        d.close()
+    e = np.abs(out[2] - out[1]).max() / np.abs(out[1]).max()
+    print(f"history across mode changes: max rel diff {e:.1e}", flush=True)
+    assert e < 1e-9
