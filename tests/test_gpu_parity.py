@@ -228,6 +228,37 @@ def test_full_size_properties(gpu, nx, nz, k0, seed, dt):
     assert np.linalg.norm(outs[2] - outs[0]) / np.linalg.norm(outs[0]) < TOL32
 
 
+def test_full_size_coupled_properties(gpu):
+    """The coupled step at the precision configuration's full size
+    (1024²×32, Monge-Ampère solve per RK stage with the extrapolated warm
+    start, dt = 2.5e-4): every stage solve converges, Σb is conserved to
+    roundoff, a rerun is bitwise identical, and the FP32-mixed form stays
+    within the metric's tolerance of the FP64 one."""
+    import ctypes
+    from ssgamd.ics import spectral_b0
+    nx, nz, dt, steps = 1024, 32, 2.5e-4, 20
+    b0 = spectral_b0(nx, nx, seed=0, k0=6.0)
+    outs = []
+    for prec in (0, 0, 1):
+        d = gpu.make_domain(nx, nx, nz, precision=prec)
+        f = gpu.allocate_fields(d)
+        f.bs = b0
+        gpu.set_coupling(d, eps=0.1, tol=1e-10, maxiter=60, variant=gpu.MA_BODY)
+        st = gpu.TimeState(0.0, f)
+        gpu.timestep(f, d, gpu.TimeParams(dt, scheme=gpu.RK3, filter_freq=10, filter_strength=36.0,
+                                          filter_order=8, filter_cutoff=0.65), st, steps)
+        last = gpu._MaDiag()
+        d.check(d.lib.ssg_last_ma_diag(d.handle, ctypes.byref(last)))
+        assert last.converged and last.relative_residual <= 1e-10
+        outs.append(f.bs)
+        d.close()
+    assert np.array_equal(outs[0], outs[1])
+    assert abs(outs[0].sum() - b0.sum()) < 1e-9
+    e = np.linalg.norm(outs[2] - outs[0]) / np.linalg.norm(outs[0])
+    print(f"coupled 1024^2x32, {steps} steps: FP32-mixed vs FP64 b rel-L2 {e:.1e}", flush=True)
+    assert e < TOL32
+
+
 def test_graph_replay_matches_eager(gpu, monkeypatch):
     """The captured filter-cycle graph, replayed many times with eager work and
     synchronisations in between, gives bitwise the eager launches' result."""
