@@ -99,7 +99,7 @@ def cpu_baseline(b0, psi, nx, ny, nz, dt, budget_s):
             "sample": f"{n} RK3 steps of the same {nx}x{ny} workload (FP64 C oracle, OpenMP), {el:.1f} s"}
 
 
-def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2):
+def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2, step_tol=1e-10):
     """Coupled step (Monge-Ampère solve at every RK stage, body-forced
     variant, ε = 0.1, relative residual 1e-10): steps/s plus the Monge-Ampère
     solve's iterations, time and per-kernel-class profile, and the final
@@ -125,7 +125,7 @@ def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2):
     # restart from the cold state so both precisions step the same trajectory
     f.bs = b0
     ssg.solve_monge_ampere(f, d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY)
-    ssg.set_coupling(d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY, warm=warm)
+    ssg.set_coupling(d, eps=0.1, tol=step_tol, maxiter=60, variant=ssg.MA_BODY, warm=warm)
     st = ssg.TimeState(0.0, f)
     params = ssg.TimeParams(dt, scheme=ssg.RK3, filter_freq=10, filter_strength=36.0, filter_order=8,
                             filter_cutoff=0.65)
@@ -139,7 +139,7 @@ def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2):
     d.lib.ssg_last_ma_diag(d.handle, ctypes.byref(last))
     fields = {"b": f.bs, "phis": f.phis, "ke": ssg.diagnostics(d)["ke_surface"]}
     d.close()
-    return {"steps_per_s": steps / wall, "steps": steps, "variant": "body (as wired)", "eps": 0.1, "tol": 1e-10,
+    return {"steps_per_s": steps / wall, "steps": steps, "variant": "body (as wired)", "eps": 0.1, "tol": step_tol,
             "cold_solve": cold, "warm_start": warm, "warm_solve_last_iterations": last.iterations,
             "warm_solve_last_iterations_fp32": last.iterations_fp32,
             "warm_solve_last_relative_residual": last.relative_residual, "ma_kernels": kernels}, fields
@@ -445,6 +445,14 @@ def main():
                                       "phis_rel_l2": rel_l2(f32["phis"], f64["phis"]),
                                       "ke_surface_rel": abs(f32["ke"] - f64["ke"]) / abs(f64["ke"]),
                                       "steps": args.coupled_steps + 1}
+        # the FP32-mixed step with its stage solves at a tolerance matched to
+        # the FP32 tendency's own rounding, against the FP64 step at 1e-10
+        c32m, f32m = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP32_MIXED,
+                                     args.ma_warm, step_tol=1e-8)
+        out["coupled_fp32_tol1e-8"] = c32m
+        out["coupled_fp32_tol1e-8_errors"] = {"b_rel_l2": rel_l2(f32m["b"], f64["b"]),
+                                              "phis_rel_l2": rel_l2(f32m["phis"], f64["phis"]),
+                                              "ke_surface_rel": abs(f32m["ke"] - f64["ke"]) / abs(f64["ke"])}
     if args.adjoint_steps > 0 and not args.only:
         log(f"bench: adjoint configuration 256x256x16, {args.adjoint_steps} steps")
         out["adjoint"] = measure_adjoint(ssg, args.adjoint_steps)
