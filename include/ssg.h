@@ -3,8 +3,8 @@
  *
  * Drop-in for the time-step hot path of SSG.jl: the reference's
  * `timestep!(fields, domain, params, state)` and the spectral routines under
- * it.  A Julia host binds these with `ccall`; the
- * Python mirror in ssg.jl_amd/ssgamd binds them with ctypes.
+ * it.  A Julia host binds these with `ccall`; the Python mirror in
+ * ssg.jl_amd/ssgamd binds them with ctypes.
  *
  * Conventions
  *   - Real 2D fields cross the ABI as FP64, column-major with x fastest:
