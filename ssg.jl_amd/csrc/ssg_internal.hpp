@@ -89,7 +89,9 @@ struct ssg_ctx {
     cpx<double>* ma_b2 = nullptr;     // forcing spectrum [kx][ky]
     double* ma_coef = nullptr;
     double* ma_partial = nullptr;
-    double* ma_partial_host = nullptr;
+    MaSum* ma_sum_host = nullptr;     // pinned: reduced residual sums written by the device
+    MaSum* ma_sum_dev = nullptr;      // device address of ma_sum_host
+    unsigned long long ma_sum_seq = 0;
     int ma_partial_cap = 0;
     int ma_partial_used = 0;
     bool ma_have_d = false;

@@ -432,4 +432,42 @@ __global__ __launch_bounds__(128) void k_ma_tridiag_adj(MaTriAdjArgs a) {
     a.bbar[(size_t)kx * ny + ky] = bsum;
 }
 
+// Residual sums of one evaluation, in pinned host memory that the device
+// writes.  `seq` is stored last, with a system-scope release, so a host that
+// reads the expected `seq` with an acquire load also sees r2 and h2.
+struct MaSum {
+    double r2;
+    double h2;
+    unsigned long long seq;
+    unsigned long long pad;
+};
+
+// Final reduction of the per-block residual sums of an OP_MA_DIFF / OP_MA_ADJ
+// pass (two doubles per block), in a fixed order, published to `out`.
+static __global__ __launch_bounds__(256) void k_ma_sum(const double* __restrict__ part, int nb, MaSum* out,
+                                                       unsigned long long seq) {
+    __shared__ double s0[256], s1[256];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int i = t; i < nb; i += 256) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    s0[t] = a;
+    s1[t] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            s0[t] += s0[t + w];
+            s1[t] += s1[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out->r2 = s0[0];
+        out->h2 = s1[0];
+        __hip_atomic_store(&out->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 }  // namespace ssg

@@ -28,7 +28,9 @@ int ma_alloc(ssg_ctx* c) {
     }
     c->ma_partial_cap = c->ncols_full * c->nz + 64;
     HIP_OK(c, hipMalloc(&c->ma_partial, (size_t)c->ma_partial_cap * 2 * sizeof(double)));
-    HIP_OK(c, hipHostMalloc(&c->ma_partial_host, (size_t)c->ma_partial_cap * 2 * sizeof(double), hipHostMallocDefault));
+    HIP_OK(c, hipHostMalloc(&c->ma_sum_host, sizeof(MaSum), hipHostMallocMapped | hipHostMallocCoherent));
+    memset(c->ma_sum_host, 0, sizeof(MaSum));
+    HIP_OK(c, hipHostGetDevicePointer((void**)&c->ma_sum_dev, c->ma_sum_host, 0));
     HIP_OK(c, hipMalloc(&c->bst, (size_t)c->nx * c->ny * sizeof(double)));
     // stencil coefficients (uniform grid dz of the context)
     const int nz = c->nz;
@@ -70,7 +72,7 @@ void ma_free(ssg_ctx* c) {
     hist_free(c->ma_hist);
     (void)hipFree(c->ma_coef);
     (void)hipFree(c->ma_partial);
-    if (c->ma_partial_host) (void)hipHostFree(c->ma_partial_host);
+    if (c->ma_sum_host) (void)hipHostFree(c->ma_sum_host);
     (void)hipFree(c->bst);
 }
 
@@ -317,6 +319,34 @@ int ma_tune(ssg_ctx* c) {
     return rc;
 }
 
+// Sums (Σ|r̂|², Σ|b̂ + εD̂|²) of the last OP_MA_DIFF / OP_MA_ADJ pass, reduced on
+// the device and published to pinned memory.  The host polls the published
+// sequence number rather than synchronising the stream, so it resumes as
+// soon as the sums land instead of after a copy and the stream's completion
+// signal.  Every 1024 polls it queries the stream: an error ends the wait,
+// and a drained stream without the sums is an error, so the wait always ends.
+static int ma_read_sums(ssg_ctx* c, double* r2, double* h2) {
+    const unsigned long long want = ++c->ma_sum_seq;
+    hipLaunchKernelGGL(k_ma_sum, dim3(1), dim3(256), 0, c->stream, (const double*)c->ma_partial,
+                       c->ma_partial_used, c->ma_sum_dev, want);
+    HIP_OK(c, hipGetLastError());
+    for (unsigned polls = 1;; ++polls) {
+        if (__atomic_load_n(&c->ma_sum_host->seq, __ATOMIC_ACQUIRE) == want) break;
+        if (polls % 1024) {
+            __builtin_ia32_pause();
+            continue;
+        }
+        const hipError_t q = hipStreamQuery(c->stream);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) return set_err(c, SSG_ERR_HIP, "residual read-back: %s", hipGetErrorString(q));
+        if (__atomic_load_n(&c->ma_sum_host->seq, __ATOMIC_ACQUIRE) == want) break;
+        return set_err(c, SSG_ERR_HIP, "residual read-back: the sums were not published");
+    }
+    *r2 = c->ma_sum_host->r2;
+    *h2 = c->ma_sum_host->h2;
+    return SSG_OK;
+}
+
 // Residual evaluation of the iterate Φ̂ = L⁻¹(b̂ + ε D̂_prev): D̂ of Φ into
 // `dh` (which holds D̂_prev on entry) and the residual ε(D̂_prev - D̂) on
 // interior levels, reduced over blocks and read back.  `rabs` is
@@ -344,15 +374,8 @@ int ma_eval(ssg_ctx* c, const ssg_ma_params* p, const cpx<R>* phi, cpx<R>* const
     b.body = p->variant == SSG_MA_BODY ? 1 : 0;
     b.partial = c->ma_partial;
     if ((rc = launch_cols_args<R, OP_MA_DIFF>(c, b, lines, c->nz))) return rc;
-    const int nb = c->ma_partial_used;
-    HIP_OK(c, hipMemcpyAsync(c->ma_partial_host, c->ma_partial, (size_t)nb * 2 * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(c, hipStreamSynchronize(c->stream));
     double r2 = 0, h2 = 0;
-    for (int i = 0; i < nb; ++i) {
-        r2 += c->ma_partial_host[2 * i];
-        h2 += c->ma_partial_host[2 * i + 1];
-    }
+    if ((rc = ma_read_sums(c, &r2, &h2))) return rc;
     // Parseval: sum of r² over the grid = (1/(nx ny)) sum of |r̂|² per level
     *rabs = std::sqrt(r2 * scale);
     *rrel = h2 > 0 ? std::sqrt(r2 / h2) : std::sqrt(r2 * scale);
@@ -839,15 +862,8 @@ int ma_adjoint_solve(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d, bool wa
         b.eps = p->eps;
         b.partial = c->ma_partial;
         if ((rc = launch_cols_args<double, OP_MA_ADJ>(c, b, lines, c->nz))) return rc;
-        const int nb = c->ma_partial_used;
-        HIP_OK(c, hipMemcpyAsync(c->ma_partial_host, c->ma_partial, (size_t)nb * 2 * sizeof(double),
-                                 hipMemcpyDeviceToHost, c->stream));
-        HIP_OK(c, hipStreamSynchronize(c->stream));
         double r2 = 0, h2 = 0;
-        for (int i = 0; i < nb; ++i) {
-            r2 += c->ma_partial_host[2 * i];
-            h2 += c->ma_partial_host[2 * i + 1];
-        }
+        if ((rc = ma_read_sums(c, &r2, &h2))) return rc;
         d->final_residual = std::sqrt(r2 * scale);
         d->relative_residual = h2 > 0 ? std::sqrt(r2 / h2) : 0.0;
         if (it < 64) d->history[it] = d->relative_residual;
