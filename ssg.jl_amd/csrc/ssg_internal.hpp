@@ -91,7 +91,6 @@ struct ssg_ctx {
     double* ma_partial = nullptr;
     MaSum* ma_sum_host = nullptr;     // pinned: reduced residual sums written by the device
     MaSum* ma_sum_dev = nullptr;      // device address of ma_sum_host
-    unsigned long long ma_sum_seq = 0;
     int ma_partial_cap = 0;
     int ma_partial_used = 0;
     bool ma_have_d = false;

@@ -273,6 +273,7 @@ struct ColsArgs {
     int fdelta;            // filter returns (sigma - 1) * P
     const cpx<R>* twy;
     Grid g;
+    long in_stride;         // elements between the lines of inA (0: ny, contiguous lines)
     // Monge-Ampere extensions (lines are (kx, level) pairs, kx = line / nlev)
     int nlev;
     cpx<R>* outC;
@@ -336,7 +337,7 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
     C vv[2][E];
     C (&va)[E] = vv[0];
     C (&vb)[E] = vv[1];
-    const C* srcA = a.inA + (size_t)(live ? line : 0) * pitch;
+    const C* srcA = a.inA + (size_t)(live ? line : 0) * (a.in_stride ? (size_t)a.in_stride : (size_t)pitch);
 #pragma unroll
     for (int m = 0; m < E; ++m) va[m] = live ? srcA[t + T * m] : C{0, 0};
     if constexpr (TWO_IN) {

@@ -433,30 +433,45 @@ __global__ __launch_bounds__(128) void k_ma_tridiag_adj(MaTriAdjArgs a) {
 }
 
 // Residual sums of one evaluation, in pinned host memory that the device
-// writes.  `seq` is stored last, with a system-scope release, so a host that
-// reads the expected `seq` with an acquire load also sees r2 and h2.
+// writes.  The host sets both words to kMaSumUnset before the reduction is
+// launched; the device writes each with a relaxed system-scope atomic store
+// (single-copy atomic, performed at the system coherence point, so no cache
+// writeback is needed), and the host waits until neither word is unset.
+// Sums of squares are never negative, so the unset pattern (-1.0) is never a
+// result; a NaN result differs from it and is passed on.
 struct MaSum {
-    double r2;
-    double h2;
-    unsigned long long seq;
-    unsigned long long pad;
+    unsigned long long r2;  // bits of a double
+    unsigned long long h2;
 };
+constexpr unsigned long long kMaSumUnset = 0xBFF0000000000000ull;  // -1.0
 
 // Final reduction of the per-block residual sums of an OP_MA_DIFF / OP_MA_ADJ
-// pass (two doubles per block), in a fixed order, published to `out`.
-static __global__ __launch_bounds__(256) void k_ma_sum(const double* __restrict__ part, int nb, MaSum* out,
-                                                       unsigned long long seq) {
-    __shared__ double s0[256], s1[256];
+// pass (two doubles per block), in a fixed order, published to `out`.  One
+// block of kMaSumThreads; each thread's loads are issued together (unrolled
+// by 8) so the pass costs about one memory latency.
+constexpr int kMaSumThreads = 1024;
+static __global__ __launch_bounds__(kMaSumThreads) void k_ma_sum(const double2* __restrict__ part, int nb,
+                                                                 MaSum* out) {
+    __shared__ double s0[kMaSumThreads], s1[kMaSumThreads];
     const int t = threadIdx.x;
     double a = 0.0, b = 0.0;
-    for (int i = t; i < nb; i += 256) {
-        a += part[2 * i];
-        b += part[2 * i + 1];
+    for (int i0 = t; i0 < nb; i0 += 8 * kMaSumThreads) {
+        double2 v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = i0 + k * kMaSumThreads;
+            v[k] = i < nb ? part[i] : double2{0.0, 0.0};
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            a += v[k].x;
+            b += v[k].y;
+        }
     }
     s0[t] = a;
     s1[t] = b;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = kMaSumThreads / 2; w > 0; w >>= 1) {
         if (t < w) {
             s0[t] += s0[t + w];
             s1[t] += s1[t + w];
@@ -464,9 +479,10 @@ static __global__ __launch_bounds__(256) void k_ma_sum(const double* __restrict_
         __syncthreads();
     }
     if (t == 0) {
-        out->r2 = s0[0];
-        out->h2 = s1[0];
-        __hip_atomic_store(&out->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&out->r2, (unsigned long long)__double_as_longlong(s0[0]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&out->h2, (unsigned long long)__double_as_longlong(s1[0]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 

@@ -29,7 +29,7 @@ int ma_alloc(ssg_ctx* c) {
     c->ma_partial_cap = c->ncols_full * c->nz + 64;
     HIP_OK(c, hipMalloc(&c->ma_partial, (size_t)c->ma_partial_cap * 2 * sizeof(double)));
     HIP_OK(c, hipHostMalloc(&c->ma_sum_host, sizeof(MaSum), hipHostMallocMapped | hipHostMallocCoherent));
-    memset(c->ma_sum_host, 0, sizeof(MaSum));
+    c->ma_sum_host->r2 = c->ma_sum_host->h2 = kMaSumUnset;
     HIP_OK(c, hipHostGetDevicePointer((void**)&c->ma_sum_dev, c->ma_sum_host, 0));
     HIP_OK(c, hipMalloc(&c->bst, (size_t)c->nx * c->ny * sizeof(double)));
     // stencil coefficients (uniform grid dz of the context)
@@ -320,18 +320,26 @@ int ma_tune(ssg_ctx* c) {
 }
 
 // Sums (Σ|r̂|², Σ|b̂ + εD̂|²) of the last OP_MA_DIFF / OP_MA_ADJ pass, reduced on
-// the device and published to pinned memory.  The host polls the published
-// sequence number rather than synchronising the stream, so it resumes as
-// soon as the sums land instead of after a copy and the stream's completion
-// signal.  Every 1024 polls it queries the stream: an error ends the wait,
-// and a drained stream without the sums is an error, so the wait always ends.
+// the device and published to pinned memory (struct MaSum).  The host polls
+// the two words rather than copying the partials and synchronising the
+// stream, so it resumes as soon as the sums land.  Every 1024 polls it
+// queries the stream: an error ends the wait, and a drained stream without
+// the sums is an error, so the wait always ends.
 static int ma_read_sums(ssg_ctx* c, double* r2, double* h2) {
-    const unsigned long long want = ++c->ma_sum_seq;
-    hipLaunchKernelGGL(k_ma_sum, dim3(1), dim3(256), 0, c->stream, (const double*)c->ma_partial,
-                       c->ma_partial_used, c->ma_sum_dev, want);
+    MaSum* h = c->ma_sum_host;
+    __atomic_store_n(&h->r2, kMaSumUnset, __ATOMIC_RELAXED);
+    __atomic_store_n(&h->h2, kMaSumUnset, __ATOMIC_RELAXED);
+    __atomic_thread_fence(__ATOMIC_SEQ_CST);
+    hipLaunchKernelGGL(k_ma_sum, dim3(1), dim3(kMaSumThreads), 0, c->stream, (const double2*)c->ma_partial,
+                       c->ma_partial_used, c->ma_sum_dev);
     HIP_OK(c, hipGetLastError());
-    for (unsigned polls = 1;; ++polls) {
-        if (__atomic_load_n(&c->ma_sum_host->seq, __ATOMIC_ACQUIRE) == want) break;
+    unsigned long long a = kMaSumUnset, b = kMaSumUnset;
+    auto ready = [&]() {
+        a = __atomic_load_n(&h->r2, __ATOMIC_ACQUIRE);
+        b = __atomic_load_n(&h->h2, __ATOMIC_ACQUIRE);
+        return a != kMaSumUnset && b != kMaSumUnset;
+    };
+    for (unsigned polls = 1; !ready(); ++polls) {
         if (polls % 1024) {
             __builtin_ia32_pause();
             continue;
@@ -339,11 +347,11 @@ static int ma_read_sums(ssg_ctx* c, double* r2, double* h2) {
         const hipError_t q = hipStreamQuery(c->stream);
         if (q == hipErrorNotReady) continue;
         if (q != hipSuccess) return set_err(c, SSG_ERR_HIP, "residual read-back: %s", hipGetErrorString(q));
-        if (__atomic_load_n(&c->ma_sum_host->seq, __ATOMIC_ACQUIRE) == want) break;
+        if (ready()) break;
         return set_err(c, SSG_ERR_HIP, "residual read-back: the sums were not published");
     }
-    *r2 = c->ma_sum_host->r2;
-    *h2 = c->ma_sum_host->h2;
+    memcpy(r2, &a, sizeof(double));
+    memcpy(h2, &b, sizeof(double));
     return SSG_OK;
 }
 
@@ -494,12 +502,13 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
 // (src/setICs.jl:161-192), taken from the top level of Φ̂
 int ma_surface_outputs(ssg_ctx* c) {
     PathBuffers<double>& D = c->d;
-    const size_t lb = (size_t)c->ny * sizeof(cpx<double>);
-    HIP_OK(c, hipMemcpy2DAsync(D.Tp, lb, c->ma_phi + (size_t)(c->nz - 1) * c->ny, lb * c->nz, lb, c->ncols_full,
-                               hipMemcpyDeviceToDevice, c->stream));
     const double scale = 1.0 / ((double)c->nx * c->ny);
+    // the top level's lines are read in place: kx stride nz * ny in Φ̂
+    const cpx<double>* top = c->ma_phi + (size_t)(c->nz - 1) * c->ny;
+    const long top_stride = (long)c->nz * c->ny;
     ColsArgs<double> a{};
-    a.inA = D.Tp;
+    a.inA = top;
+    a.in_stride = top_stride;
     a.outA = D.Fx;
     a.outB = D.Fy;
     a.scale = scale;
@@ -511,7 +520,12 @@ int ma_surface_outputs(ssg_ctx* c) {
     w.ncols_in = c->ncols_full;
     w.w_out = D.w;
     if ((rc = launch_rows_inv<double, MODE_PLANES, EPI_STORE_W>(c, w))) return rc;
-    if ((rc = launch_cols<double, OP_INV>(c, D.Tp, nullptr, D.Tp, nullptr, c->ncols_full, scale))) return rc;
+    ColsArgs<double> v{};
+    v.inA = top;
+    v.in_stride = top_stride;
+    v.outA = D.Tp;
+    v.scale = scale;
+    if ((rc = launch_cols_args<double, OP_INV>(c, v, c->ncols_full, 1))) return rc;
     RowsInvArgs<double> f{};
     f.P0 = D.Tp;
     f.ncols_in = c->ncols_full;
