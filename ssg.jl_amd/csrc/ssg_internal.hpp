@@ -406,6 +406,9 @@ inline int prof_end(ssg_ctx* c, int slot, int kind, double bytes) {
     return SSG_OK;
 }
 
+// one thread per element, 256-thread blocks
+inline unsigned grid_all(long n) { return (unsigned)((n + 255) / 256); }
+
 inline int grid1d(long n) {
     long b = (n + 255) / 256;
     if (b > 4096) b = 4096;

@@ -255,6 +255,30 @@ __global__ __launch_bounds__(256) void k_rows_fwd(RowsFwdArgs<R, S> a) {
     rows_fwd_body<R, S, N, E, SRC>(a, xcd_block(blockIdx.x, a.nblk));
 }
 
+// Access to the Monge-Ampère planes.  With `nt` set (planes larger than the
+// caches: each pass streams them once) loads and stores are nontemporal;
+// otherwise plain, so planes that fit the caches stay there between passes.
+template <typename R> struct vec2_t;
+template <> struct vec2_t<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct vec2_t<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <typename R>
+__device__ __forceinline__ cpx<R> ld_plane(const cpx<R>* p, bool nt) {
+    using V = typename vec2_t<R>::type;
+    if (nt) {
+        const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+        return cpx<R>{v.x, v.y};
+    }
+    return *p;
+}
+template <typename R>
+__device__ __forceinline__ void st_plane(cpx<R>* p, cpx<R> x, bool nt) {
+    using V = typename vec2_t<R>::type;
+    if (nt)
+        __builtin_nontemporal_store(V{x.x, x.y}, reinterpret_cast<V*>(p));
+    else
+        *p = x;
+}
+
 // ---------------------------------------------------------------------------
 // Column pass: per kx column, y-FFT -> symbol -> inverse y-FFT.
 // ---------------------------------------------------------------------------
@@ -274,6 +298,7 @@ struct ColsArgs {
     const cpx<R>* twy;
     Grid g;
     long in_stride;         // elements between the lines of inA (0: ny, contiguous lines)
+    int nt;                 // OP_MA_DERIV / OP_MA_DIFF: nontemporal plane access (ld_plane)
     // Monge-Ampere extensions (lines are (kx, level) pairs, kx = line / nlev)
     int nlev;
     cpx<R>* outC;
@@ -338,8 +363,14 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
     C (&va)[E] = vv[0];
     C (&vb)[E] = vv[1];
     const C* srcA = a.inA + (size_t)(live ? line : 0) * (a.in_stride ? (size_t)a.in_stride : (size_t)pitch);
+    constexpr bool MA_PLANES = (OP == OP_MA_DERIV || OP == OP_MA_DIFF);
 #pragma unroll
-    for (int m = 0; m < E; ++m) va[m] = live ? srcA[t + T * m] : C{0, 0};
+    for (int m = 0; m < E; ++m) {
+        if constexpr (MA_PLANES)
+            va[m] = live ? ld_plane<R>(srcA + t + T * m, a.nt) : C{0, 0};
+        else
+            va[m] = live ? srcA[t + T * m] : C{0, 0};
+    }
     if constexpr (TWO_IN) {
         const C* srcB = a.inB + (size_t)(live ? line : 0) * pitch;
 #pragma unroll
@@ -349,7 +380,12 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
         if (live) {
             C* d = dst + (size_t)line * pitch;
 #pragma unroll
-            for (int m = 0; m < E; ++m) d[t + T * m] = v[m];
+            for (int m = 0; m < E; ++m) {
+                if constexpr (MA_PLANES)
+                    st_plane<R>(d + t + T * m, v[m], a.nt);
+                else
+                    d[t + T * m] = v[m];
+            }
         }
     };
 
@@ -458,7 +494,7 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
             for (int m = 0; m < E; ++m) {
                 const int y = t + T * m;
                 if (live) {
-                    const C prev = dp[y];
+                    const C prev = ld_plane<R>(dp + y, a.nt);
                     if (interior) {
                         const double rx = a.eps * ((double)prev.x - (double)va[m].x);
                         const double ry = a.eps * ((double)prev.y - (double)va[m].y);
@@ -471,7 +507,7 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
                         acc[0] += wgt * (rx * rx + ry * ry);
                         acc[1] += wgt * (hx * hx + hy * hy);
                     }
-                    dp[y] = va[m];
+                    st_plane<R>(dp + y, va[m], a.nt);
                 }
             }
             block_sum_g<2>(acc, a.partial, bidx);
@@ -929,20 +965,22 @@ static __global__ __launch_bounds__(256) void k_wmax(const double2* __restrict__
     }
 }
 
-// warm-start extrapolation: out = ca a + cb b + cc c (c may be null)
-static __global__ void k_extrap(double2* __restrict__ out, const double2* __restrict__ a, double ca,
-                                const double2* __restrict__ b, double cb, const double2* __restrict__ c, double cc,
-                                long n) {
-    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const double2 x = a[q], y = b[q];
-        double2 r = {ca * x.x + cb * y.x, ca * x.y + cb * y.y};
-        if (c) {
-            const double2 z = c[q];
-            r.x += cc * z.x;
-            r.y += cc * z.y;
-        }
-        out[q] = r;
+// warm-start extrapolation: out = ca a + cb b + cc c (c may be null).  One
+// element per thread (grid of ceil(n / 256) blocks); plane access as in
+// ld_plane / st_plane.
+static __global__ __launch_bounds__(256) void k_extrap(double2* __restrict__ out, const double2* __restrict__ a,
+                                                       double ca, const double2* __restrict__ b, double cb,
+                                                       const double2* __restrict__ c, double cc, long n, int nt) {
+    const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double2 x = ld_plane<double>(a + q, nt), y = ld_plane<double>(b + q, nt);
+    double2 r = {ca * x.x + cb * y.x, ca * x.y + cb * y.y};
+    if (c) {
+        const double2 z = ld_plane<double>(c + q, nt);
+        r.x += cc * z.x;
+        r.y += cc * z.y;
     }
+    st_plane<double>(out + q, r, nt);
 }
 
 // out = a x + b y (y may be null)

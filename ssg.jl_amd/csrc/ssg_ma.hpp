@@ -221,6 +221,7 @@ struct MaTriArgs {
     int body;                   // B2 on interior rows (variant A)
     int top;                    // B2 as top Neumann data (variant B)
     int use_d;                  // 0: D term omitted (first solve without history)
+    int nt;                     // nontemporal plane access (ld_plane)
     Grid g;
 };
 
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(128) void k_ma_tridiag(MaTriArgs<R, P> a) {
         double2 f = {0.0, 0.0};
         if (a.body) f = b2;
         if (a.use_d) {
-            const cpx<R> d = a.Dh[base + (size_t)k * ny];
+            const cpx<R> d = ld_plane<R>(a.Dh + base + (size_t)k * ny, a.nt);
             f.x += a.eps * (double)d.x;
             f.y += a.eps * (double)d.y;
         }
@@ -305,7 +306,7 @@ __global__ __launch_bounds__(128) void k_ma_tridiag(MaTriArgs<R, P> a) {
     for (int k = NZMAX - 1; k >= 0; --k) {
         if (k <= nz - 1) {
             x = {dp[k].x - cp[k] * x.x, dp[k].y - cp[k] * x.y};
-            a.Phi[base + (size_t)k * ny] = cpx<P>{(P)x.x, (P)x.y};
+            st_plane<P>(a.Phi + base + (size_t)k * ny, cpx<P>{(P)x.x, (P)x.y}, a.nt);
         }
     }
 }
