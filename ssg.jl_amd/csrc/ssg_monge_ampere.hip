@@ -9,6 +9,15 @@ void hist_free(SolveHistory& h);
 // ---------------------------------------------------------------------------
 // Monge-Ampère solve (FP64)
 // ---------------------------------------------------------------------------
+// Nontemporal access to the Monge-Ampère planes (ld_plane) pays when an
+// FP64 plane (C3) exceeds kMaStreamBytes: then the passes stream planes that
+// the caches cannot hold.  Smaller planes stay cache-resident between passes
+// with plain access (measured: the 256²×16 coupled gradient is 7 % slower
+// with nontemporal access, the 1024²×32 coupled step 2 % faster).  The
+// environment variable SSG_MA_STREAM=0/1, read when a context first solves,
+// overrides the choice; both give bitwise the same results.
+constexpr size_t kMaStreamBytes = 128u << 20;
+
 int ma_alloc(ssg_ctx* c) {
     if (c->ma_ready) return SSG_OK;
     if (c->nz < 3 || c->nz > 128)
@@ -18,6 +27,10 @@ int ma_alloc(ssg_ctx* c) {
     if (3 * (size_t)lds_line_elems(c->ny) * sizeof(cpx<double>) > 160 * 1024)
         return set_err(c, SSG_ERR_UNSUPPORTED, "the Monge-Ampere solve supports ny <= 2048 (ny = %d)", c->ny);
     const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    if (const char* env = getenv("SSG_MA_STREAM"))
+        c->ma_stream = atoi(env) != 0 ? 1 : 0;
+    else
+        c->ma_stream = c3 * sizeof(cpx<double>) > kMaStreamBytes ? 1 : 0;
     HIP_OK(c, hipMalloc(&c->ma_phi, c3 * sizeof(cpx<double>)));
     HIP_OK(c, hipMalloc(&c->ma_dprev, c3 * sizeof(cpx<double>)));
     for (int i = 0; i < 3; ++i) HIP_OK(c, hipMalloc(&c->ma_p[i], c3 * sizeof(cpx<double>)));
@@ -148,15 +161,7 @@ int hist_push(ssg_ctx* c, SolveHistory& h, cpx<double>*& work, int period) {
     return SSG_OK;
 }
 
-// Nontemporal access to the Monge-Ampère planes (ld_plane) pays when an
-// FP64 plane (C3) exceeds kMaStreamBytes: then the passes stream planes that
-// the caches cannot hold.  Smaller planes stay cache-resident between passes
-// with plain access (measured: the 256²×16 coupled gradient is 7 % slower
-// with nontemporal access, the 1024²×32 coupled step 2 % faster).
-constexpr size_t kMaStreamBytes = 128u << 20;
-static int ma_stream_planes(const ssg_ctx* c) {
-    return (size_t)c->ncols_full * c->nz * c->ny * sizeof(cpx<double>) > kMaStreamBytes ? 1 : 0;
-}
+static int ma_stream_planes(const ssg_ctx* c) { return c->ma_stream; }
 
 // Extrapolated first iterate into the working buffer (needs h.n >= 1).  With
 // a stepping period P (solves per step) and three steps of history, the
