@@ -122,7 +122,9 @@ typedef struct {
     double history[64];        /* relative residual per iteration (first 64) */
     double solve_ms;
     int32_t iterations_fp32;   /* leading iterations run in FP32 arithmetic (FP32-mixed contexts) */
-    int32_t reserved;
+    int32_t not_elliptic;      /* 1: body variant with eps*max(b) >= 1, rejected before iterating: at its
+                                  z-independent fixed point det(I - eps*H) = 1 - eps*b (H the horizontal
+                                  Hessian of Φ), so no elliptic solution exists */
 } ssg_ma_diag;
 
 /* make_domain + allocate_fields + TimeState (src/domain.jl:217-230,
@@ -145,6 +147,14 @@ int ssg_get_field_device(ssg_ctx* ctx, int32_t field, double* dev, int64_t n);
    ->dt_actual.  SSG_RK3_LOWSTORAGE returns SSG_ERR_UNSUPPORTED, as the
    reference raises (src/timestep.jl:275-277). */
 int ssg_timestep(ssg_ctx* ctx, const ssg_time_params* params, ssg_time_state* state, int64_t nsteps);
+/* Set-up half of ssg_timestep (no reference counterpart; the reference has
+   no launch set-up): runs the autotuner and instantiates the hipGraphs that
+   ssg_timestep(ctx, params, state, nsteps) will replay from this state,
+   without advancing it.  Step runs are replayed as graphs of up to one
+   filter cycle each, keyed by their phase in the cycle and their length, so
+   a prepared call launches no capture.  ssg_timestep prepares by itself;
+   calling this first keeps the capture out of a timed call. */
+int ssg_timestep_prepare(ssg_ctx* ctx, const ssg_time_params* params, const ssg_time_state* state, int64_t nsteps);
 
 /* compute_tendency! (src/nonlinear.jl:22-36) of the current b, φₛ into
    out (nx*ny).  ab2_reference_aliasing = 1 gives the tendency the AB2 call
@@ -170,7 +180,9 @@ int ssg_diagnostics(ssg_ctx* ctx, ssg_diag* out);
    Neumann rows of apply_ssg_boundary_conditions! (src/poisson.jl:262-364)
    for the current bₛ, then sets φₛ = Φ[:,:,Nz] (extract_surface_to_2d!,
    src/poisson.jl:1119-1131) and the surface velocities.  Needs nz >= 3.
-   Returns SSG_ERR_NOT_CONVERGED (with diag filled) when maxiter is hit. */
+   Returns SSG_ERR_NOT_CONVERGED (with diag filled) when maxiter is hit, when
+   the iteration diverges, and, without iterating, when a cold solve of the
+   body variant has eps*max(b) >= 1 (diag->not_elliptic = 1). */
 int ssg_ma_solve(ssg_ctx* ctx, const ssg_ma_params* params, ssg_ma_diag* diag);
 /* fields.φ (nx*ny*nz, x fastest then y then z) after ssg_ma_solve */
 int ssg_get_phi3d(ssg_ctx* ctx, double* host, int64_t n);

@@ -101,6 +101,9 @@ int ma_solve_dev(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_di
 
 // Coupled step: the Monge-Ampère problem is re-solved for the stage
 // buoyancy before every tendency evaluation, so φₛ and (u, v) follow b.
+// Each step ends with the solve for the buoyancy it returns, which is also
+// the next step's first-stage solve: after the call φₛ, u and v belong to
+// the returned b, and a following step reuses that solve (ma_phi_current).
 template <typename R>
 int run_steps_coupled(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, int64_t nsteps) {
     const double dt = p->dt;
@@ -113,27 +116,34 @@ int run_steps_coupled(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, i
         ~PeriodScope() { c->ma_period = 0; }
     } period_scope{c};
     c->ma_period = p->scheme == SSG_RK3 ? 3 : 1;
-    auto stage_tendency = [&](const double* bsrc) -> int {
+    auto solve = [&](const double* bsrc) -> int {
+        c->ma_phi_current = false;
         int r = ma_solve_dev(c, bsrc, &c->ma_par, &d);
         if (r) return r;
         if (!d.converged)
-            return set_err(c, SSG_ERR_NOT_CONVERGED, "Monge-Ampere solve did not converge (rel. residual %.3e)",
-                           d.relative_residual);
+            return ma_not_converged(c, d);
+        return SSG_OK;
+    };
+    auto stage_tendency = [&](const double* bsrc, bool solved) -> int {
+        int r;
+        if (!solved && (r = solve(bsrc))) return r;
         if ((r = flux_forward<R>(c, c->ncols_tend, bsrc))) return r;
         return tendency_cols<R>(c, p->scheme == SSG_AB2_LOWSTORAGE ? p->ab2_reference_aliasing : 0);
     };
     for (int64_t it = 0; it < nsteps; ++it) {
         const int64_t step_after = s->step + 1;
         const bool filt = p->filter_freq > 0 && (step_after % p->filter_freq) == 0;
+        // the solve for b itself ended the previous step
+        const bool solved = c->ma_phi_current;
         if (p->scheme == SSG_RK3) {
-            if ((rc = stage_tendency(c->b))) return rc;
+            if ((rc = stage_tendency(c->b, solved))) return rc;
             if ((rc = stage_inverse<R, EPI_RK1>(c, dt / 3.0, 0, 0, c->bst))) return rc;
-            if ((rc = stage_tendency(c->bst))) return rc;
+            if ((rc = stage_tendency(c->bst, false))) return rc;
             if ((rc = stage_inverse<R, EPI_RK2>(c, 2.0 * dt / 3.0, 0, 0, c->bst))) return rc;
-            if ((rc = stage_tendency(c->bst))) return rc;
+            if ((rc = stage_tendency(c->bst, false))) return rc;
             if ((rc = stage_inverse<R, EPI_RK3>(c, dt / 4.0, 0, 0))) return rc;
         } else {
-            if ((rc = stage_tendency(c->b))) return rc;
+            if ((rc = stage_tendency(c->b, solved))) return rc;
             const int first = (s->step == 0) ? 1 : 0;
             if ((rc = stage_inverse<R, EPI_AB2>(c, dt, 0, first))) return rc;
             c->ab2_have_history = true;
@@ -142,6 +152,8 @@ int run_steps_coupled(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, i
         s->step = step_after;
         s->dt_actual = dt;
         if (filt && (rc = filter_b<R>(c, p->filter_strength, p->filter_order, p->filter_cutoff, 0))) return rc;
+        if ((rc = solve(c->b))) return rc;
+        c->ma_phi_current = true;
     }
     return SSG_OK;
 }
@@ -178,25 +190,38 @@ int issue_step(ssg_ctx* c, const ssg_time_params* p, int64_t step_before, bool l
 }
 
 
-// A hipGraph of one filter cycle (filter_freq steps, the filter on the last;
-// 16 steps when the filter is off), captured on first use and replayed for
-// every aligned cycle.  Kernel arguments are fixed by the parameters, so the
-// graph is rebuilt when they change.
+// Step runs are replayed from hipGraphs.  A graph holds `len` consecutive
+// steps that start at phase `phase` of the filter cycle (phase = step % F,
+// F = filter_freq, or 16 when the filter is off) and stay within one cycle,
+// so every run of steps is a chain of at most one partial cycle, whole
+// cycles and one partial cycle, whatever its length and starting step.
+// Graphs are captured on first use (or by ssg_timestep_prepare) and kept per
+// (phase, len); kernel arguments are fixed by the parameters, so all of them
+// are dropped when the parameters change.  The AB2 step from step 0 (forward
+// Euler) runs eagerly.
+inline int cycle_len(const ssg_time_params* p) { return p->filter_freq > 0 ? p->filter_freq : 16; }
+
 template <typename R>
-int cycle_graph(ssg_ctx* c, const ssg_time_params* p, int len, int64_t step0, hipGraphExec_t* out) {
+int step_graph(ssg_ctx* c, const ssg_time_params* p, int phase, int len, int64_t step0, hipGraphExec_t* out) {
     const int path = std::is_same<R, float>::value ? 1 : 0;
     auto& gc = c->gcache[path];
-    if (gc.exec && gc.len == len && same_params(gc.p, *p)) {
-        *out = gc.exec;
-        return SSG_OK;
+    if (!gc.have_p || !same_params(gc.p, *p)) {
+        for (auto& kv : gc.exec) (void)hipGraphExecDestroy(kv.second);
+        gc.exec.clear();
+        gc.p = *p;
+        gc.have_p = true;
     }
-    if (gc.exec) {
-        (void)hipGraphExecDestroy(gc.exec);
-        gc.exec = nullptr;
+    const auto key = std::make_pair(phase, len);
+    auto it = gc.exec.find(key);
+    if (it != gc.exec.end()) {
+        *out = it->second;
+        return SSG_OK;
     }
     HIP_OK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = SSG_OK;
+    const int64_t l0 = c->launches;
     for (int i = 0; i < len && rc == SSG_OK; ++i) rc = issue_step<R>(c, p, step0 + i, false);
+    c->launches = l0;  // captured, not launched
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (rc) {
@@ -208,45 +233,78 @@ int cycle_graph(ssg_ctx* c, const ssg_time_params* p, int len, int64_t step0, hi
     e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return set_err(c, SSG_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-    gc.exec = ex;
-    gc.len = len;
-    gc.p = *p;
+    gc.exec.emplace(key, ex);
     *out = ex;
+    return SSG_OK;
+}
+
+inline bool graphs_enabled(const ssg_ctx* c) {
+    const char* genv = getenv("SSG_GRAPH");
+    return !(genv && genv[0] == '0') && !c->prof;
+}
+
+// Walk the pieces of a run of nsteps steps from step `step`: fn(step, n,
+// graph) for a piece of n steps replayed from a graph, or n = 1 and graph =
+// false for an eager step.
+template <typename Fn>
+int for_each_piece(const ssg_ctx* c, const ssg_time_params* p, int64_t step, int64_t nsteps, Fn&& fn) {
+    const bool graphs = graphs_enabled(c);
+    const int F = cycle_len(p);
+    int64_t it = 0;
+    while (it < nsteps) {
+        const bool ab2_first = p->scheme == SSG_AB2_LOWSTORAGE && step == 0;
+        int64_t n = 1;
+        bool g = false;
+        if (graphs && !ab2_first) {
+            const int phase = (int)(step % F);
+            n = std::min<int64_t>(F - phase, nsteps - it);
+            g = true;
+        }
+        int rc = fn(step, n, g, it + n == nsteps);
+        if (rc) return rc;
+        step += n;
+        it += n;
+    }
     return SSG_OK;
 }
 
 template <typename R>
 int run_steps(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, int64_t nsteps) {
     if (c->coupled) return run_steps_coupled<R>(c, p, s, nsteps);
+    c->ma_phi_current = false;
     int rc = flux_forward<R>(c, c->ncols_tend);
     if (rc) return rc;
-    const char* genv = getenv("SSG_GRAPH");
-    const bool graphs = !(genv && genv[0] == '0') && !c->prof;
-    const int F = p->filter_freq > 0 ? p->filter_freq : 16;
-    int64_t it = 0;
-    while (it < nsteps) {
-        const int64_t remaining = nsteps - it;
-        const bool aligned = p->filter_freq <= 0 || (s->step % F) == 0;
-        const bool ab2_first = p->scheme == SSG_AB2_LOWSTORAGE && s->step == 0;
-        int adv = 1;
-        if (graphs && aligned && !ab2_first && remaining > F) {
+    const int F = cycle_len(p);
+    return for_each_piece(c, p, s->step, nsteps, [&](int64_t step, int64_t n, bool graph, bool last) -> int {
+        int r;
+        if (graph) {
             hipGraphExec_t ex = nullptr;
-            if ((rc = cycle_graph<R>(c, p, F, s->step, &ex))) return rc;
+            if ((r = step_graph<R>(c, p, (int)(step % F), (int)n, step, &ex))) return r;
             HIP_OK(c, hipGraphLaunch(ex, c->stream));
-            c->launches += (p->scheme == SSG_RK3 ? 6 : 2) * F + (p->filter_freq > 0 ? 3 : 0);
-            adv = F;
-        } else {
-            if ((rc = issue_step<R>(c, p, s->step, it == nsteps - 1))) return rc;
+            for (int64_t k = step; k < step + n; ++k)
+                c->launches += (p->scheme == SSG_RK3 ? 6 : 2) +
+                               ((p->filter_freq > 0 && (k + 1) % p->filter_freq == 0) ? 3 : 0);
+        } else if ((r = issue_step<R>(c, p, step, last))) {
+            return r;
         }
-        for (int i = 0; i < adv; ++i) {
-            s->t += p->dt;
-            s->step += 1;
-        }
+        for (int64_t k = 0; k < n; ++k) s->t += p->dt;  // the same sum as one step at a time
+        s->step += n;
         s->dt_actual = p->dt;
         if (p->scheme == SSG_AB2_LOWSTORAGE) c->ab2_have_history = true;
-        it += adv;
-    }
-    return SSG_OK;
+        return SSG_OK;
+    });
+}
+
+// Instantiate the graphs that run_steps would replay for nsteps from `step`.
+template <typename R>
+int prepare_steps(ssg_ctx* c, const ssg_time_params* p, int64_t step, int64_t nsteps) {
+    if (c->coupled) return SSG_OK;
+    const int F = cycle_len(p);
+    return for_each_piece(c, p, step, nsteps, [&](int64_t st, int64_t n, bool graph, bool) -> int {
+        if (!graph) return SSG_OK;
+        hipGraphExec_t ex = nullptr;
+        return step_graph<R>(c, p, (int)(st % F), (int)n, st, &ex);
+    });
 }
 
 
@@ -441,7 +499,7 @@ int ssg_destroy(ssg_ctx* c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto& g : c->gcache)
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        for (auto& kv : g.exec) (void)hipGraphExecDestroy(kv.second);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SSG_OK;
@@ -468,6 +526,7 @@ static int field_io(ssg_ctx* c, int32_t field, const double* src, double* dst, i
             HIP_OK(c, hipMemcpyAsync(c->b, src, bytes, kind, c->stream));
             c->ab2_have_history = false;
         } else if (field == SSG_FIELD_PHI) {
+            c->ma_phi_current = false;
             HIP_OK(c, hipMemcpyAsync(c->phi, src, bytes, kind, c->stream));
             int rc = compute_velocities(c);
             if (rc) return rc;
@@ -515,8 +574,7 @@ int ssg_get_field_device(ssg_ctx* c, int32_t field, double* dev, int64_t n) {
     return field_io(c, field, nullptr, dev, n, true);
 }
 
-int ssg_timestep(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, int64_t nsteps) {
-    if (!c) return SSG_ERR_ARG;
+static int check_step(ssg_ctx* c, const ssg_time_params* p, const ssg_time_state* s, int64_t nsteps) {
     if (!p || !s) return set_err(c, SSG_ERR_ARG, "null params/state");
     if (nsteps < 0) return set_err(c, SSG_ERR_ARG, "nsteps < 0");
     if (p->scheme == SSG_RK3_LOWSTORAGE)
@@ -534,10 +592,34 @@ int ssg_timestep(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, int64_
     if (p->scheme == SSG_AB2_LOWSTORAGE && s->step > 0 && !c->ab2_have_history)
         return set_err(c, SSG_ERR_ARG, "AB2 step %lld needs the previous tendency; none is held", (long long)s->step);
     HIP_OK(c, hipSetDevice(c->device));
-    if (nsteps == 0) return SSG_OK;
+    return SSG_OK;
+}
+
+// autotune and instantiate the step graphs of nsteps from s->step
+static int prepare(ssg_ctx* c, const ssg_time_params* p, const ssg_time_state* s, int64_t nsteps) {
+    if (c->precision == SSG_FP32_MIXED) {
+        int rc = autotune<float>(c);
+        return rc ? rc : prepare_steps<float>(c, p, s->step, nsteps);
+    }
+    int rc = autotune<double>(c);
+    return rc ? rc : prepare_steps<double>(c, p, s->step, nsteps);
+}
+
+int ssg_timestep_prepare(ssg_ctx* c, const ssg_time_params* p, const ssg_time_state* s, int64_t nsteps) {
+    if (!c) return SSG_ERR_ARG;
+    int rc = check_step(c, p, s, nsteps);
+    if (rc || nsteps == 0) return rc;
+    return prepare(c, p, s, nsteps);
+}
+
+int ssg_timestep(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, int64_t nsteps) {
+    if (!c) return SSG_ERR_ARG;
     {
-        int trc = c->precision == SSG_FP32_MIXED ? autotune<float>(c) : autotune<double>(c);
-        if (trc) return trc;
+        int crc = check_step(c, p, s, nsteps);
+        if (crc || nsteps == 0) return crc;
+        // graphs are captured before the timed region, so last_timing holds
+        // kernel time only
+        if ((crc = prepare(c, p, s, nsteps))) return crc;
     }
     const int64_t l0 = c->launches;
     HIP_OK(c, hipEventRecord(c->ev0, c->stream));
@@ -828,6 +910,7 @@ int ssg_apply_exponential_filter(ssg_ctx* c, double strength, int32_t order, dou
     if (order < 1) return set_err(c, SSG_ERR_ARG, "Filter order must be positive");
     if (!(cutoff > 0 && cutoff <= 1)) return set_err(c, SSG_ERR_ARG, "Cutoff must be between 0 and 1");
     HIP_OK(c, hipSetDevice(c->device));
+    c->ma_phi_current = false;
     int rc = c->precision == SSG_FP32_MIXED ? filter_b<float>(c, strength, order, cutoff, 0)
                                             : filter_b<double>(c, strength, order, cutoff, 0);
     if (rc) return rc;

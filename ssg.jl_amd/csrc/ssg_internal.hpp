@@ -95,6 +95,7 @@ struct ssg_ctx {
     int ma_partial_cap = 0;
     int ma_partial_used = 0;
     bool ma_have_d = false;
+    bool ma_phi_current = false;      // coupled stepping: φₛ, u, v were solved for the current b with ma_par
     SolveHistory ma_hist;             // D̂ of the last converged solves (working buffer: ma_dprev)
     int ma_period = 0;                // solves per step of the running coupled stepper (RK3 3, AB2 1), else 0
     double ma_last_r0 = 1e300;        // first relative residual of the last solve
@@ -129,9 +130,11 @@ struct ssg_ctx {
     ssg_ma_diag adjc_diag{};
     SolveHistory adjc_hist;           // q of the last transposed solves (working buffer: adjc_q)
     int adjc_period = 0;              // transposed solves per step of the running adjoint sweep, else 0
+    // hipGraphs of step runs within one filter cycle, keyed by (first step's
+    // phase in the cycle, number of steps), for the parameters `p`
     struct GraphCache {
-        hipGraphExec_t exec = nullptr;
-        int len = 0;
+        std::map<std::pair<int, int>, hipGraphExec_t> exec;
+        bool have_p = false;
         ssg_time_params p{};
     } gcache[2];
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -527,6 +530,14 @@ inline bool same_params(const ssg_time_params& a, const ssg_time_params& b) {
     return a.dt == b.dt && a.scheme == b.scheme && a.filter_freq == b.filter_freq &&
            a.filter_strength == b.filter_strength && a.filter_order == b.filter_order &&
            a.filter_cutoff == b.filter_cutoff && a.ab2_reference_aliasing == b.ab2_reference_aliasing;
+}
+
+// SSG_ERR_NOT_CONVERGED for a failed Monge-Ampère solve; a solve rejected as
+// not elliptic keeps the message that says so
+inline int ma_not_converged(ssg_ctx* c, const ssg_ma_diag& d) {
+    if (d.not_elliptic) return SSG_ERR_NOT_CONVERGED;
+    return set_err(c, SSG_ERR_NOT_CONVERGED, "Monge-Ampere solve did not converge after %d iterations (rel. residual %.3e)",
+                   d.iterations, d.relative_residual);
 }
 
 // shared across translation units

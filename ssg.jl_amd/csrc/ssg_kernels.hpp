@@ -940,6 +940,26 @@ __global__ void k_convert(const A* __restrict__ in, B* __restrict__ out, long n)
         out[q] = (B)in[q];
 }
 
+// per-block max of b (-inf for a block with no points; NaN propagates)
+static __global__ __launch_bounds__(256) void k_bmax(const double* __restrict__ b, long n, double* partial) {
+    __shared__ double red[256];
+    double m = -INFINITY;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const double x = b[q];
+        m = (x > m || x != x) ? x : m;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            const double o = red[threadIdx.x + st];
+            if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 // per-block max |u| and max |v| of the interleaved velocities
 static __global__ __launch_bounds__(256) void k_wmax(const double2* __restrict__ w, long n, double* partial) {
     __shared__ double red[2][256];

@@ -188,8 +188,12 @@ int hist_extrapolate(ssg_ctx* c, const SolveHistory& h, cpx<double>* work, int p
     return SSG_OK;
 }
 
+// b no longer continues the sequence of solves (it was written, or the
+// parameters changed): drop the extrapolation history, and φₛ no longer
+// counts as solved for b by the coupled stepper
 void ma_history_reset(ssg_ctx* c) {
     hist_reset(c->ma_hist, c->ma_dprev);
+    c->ma_phi_current = false;
 }
 
 // Φ̂ = L⁻¹(b̂ + ε D̂) per wavenumber (D̂ term omitted when use_d = 0)
@@ -569,17 +573,67 @@ int ma_check_params(ssg_ctx* c, const ssg_ma_params* p) {
     return SSG_OK;
 }
 
+// Ellipticity of the body-forced problem.  With b on every level and zero
+// surface slope, the Picard iterates (and so the fixed point the solver
+// seeks) are independent of z, and each level solves tr H - ε det H = b with
+// H the horizontal Hessian of Φ.  Then det(I - εH) = 1 - ε tr H + ε² det H
+// = 1 - εb pointwise, and I - εH is positive definite (the problem
+// elliptic) only where εb < 1.  So εb >= 1 at any point leaves no elliptic
+// solution: such a solve is rejected without iterating (d->not_elliptic).
+// Checked on cold solves only; a warm solve of a stepping sequence that loses
+// ellipticity diverges and is retried cold, which then checks.
+int ma_elliptic(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_diag* d, bool* ok) {
+    *ok = true;
+    if (p->variant != SSG_MA_BODY || !(p->eps > 0)) return SSG_OK;
+    const long n = (long)c->nx * c->ny;
+    const int nb = std::min(c->npartial, (int)((n + 255) / 256));
+    hipLaunchKernelGGL(k_bmax, dim3(nb), dim3(256), 0, c->stream, bs, n, c->partial);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    HIP_OK(c, hipMemcpyAsync(c->partial_host, c->partial, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    double m = -INFINITY;
+    for (int i = 0; i < nb; ++i) m = (c->partial_host[i] > m || c->partial_host[i] != c->partial_host[i]) ? c->partial_host[i] : m;
+    if (p->eps * m < 1.0) return SSG_OK;  // NaN b is left to the iteration, which reports it
+    memset(d, 0, sizeof(*d));
+    d->not_elliptic = 1;
+    d->final_residual = d->relative_residual = INFINITY;
+    *ok = false;
+    set_err(c, SSG_ERR_NOT_CONVERGED,
+            "Monge-Ampere problem not elliptic: eps*max(b) = %.6g >= 1 (det(I - eps*H) = 1 - eps*b at the "
+            "z-independent solution)",
+            p->eps * m);
+    return SSG_OK;
+}
+
 // full solve for the buoyancy `bs` (device, FP64)
 int ma_solve_dev(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_diag* d) {
     int rc = ma_alloc(c);
     if (rc) return rc;
+    bool ok = true;
+    const bool cold = !(p->warm && c->ma_have_d);
+    if (cold && (rc = ma_elliptic(c, bs, p, d, &ok))) return rc;
+    if (!ok) {
+        hist_drop(c->ma_hist);
+        c->ma_have_d = false;
+        c->last_diag = *d;
+        return SSG_OK;
+    }
     if ((rc = rfft2_device(c, bs, c->ma_b2))) return rc;
     if ((rc = ma_iterate(c, p, d))) return rc;
     if (!d->converged && p->warm) {
         // a warm start that did not lead to convergence is retried cold
-        ssg_ma_params cold = *p;
-        cold.warm = 0;
-        if ((rc = ma_iterate(c, &cold, d))) return rc;
+        ssg_ma_params cold_p = *p;
+        cold_p.warm = 0;
+        if (!cold && (rc = ma_elliptic(c, bs, p, d, &ok))) return rc;
+        if (!ok) {
+            hist_drop(c->ma_hist);
+            c->ma_have_d = false;
+            c->last_diag = *d;
+            return SSG_OK;
+        }
+        if ((rc = ma_iterate(c, &cold_p, d))) return rc;
     }
     if ((rc = ma_surface_outputs(c))) return rc;
     c->last_diag = *d;
@@ -935,8 +989,7 @@ int coupled_step_record(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_param
     for (int st = 0; st < 3; ++st) {
         if ((rc = ma_solve_dev(c, c->adjc_bst[st], mp, &d))) return rc;
         if (!d.converged)
-            return set_err(c, SSG_ERR_NOT_CONVERGED, "Monge-Ampere solve did not converge (rel. residual %.3e)",
-                           d.relative_residual);
+            return ma_not_converged(c, d);
         HIP_OK(c, hipMemcpyAsync(c->adjc_phis[st], c->ma_phi, c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice,
                                  c->stream));
         if ((rc = flux_forward<double>(c, c->ncols_tend, c->adjc_bst[st]))) return rc;
@@ -1039,8 +1092,7 @@ int coupled_step_record_ab2(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_p
     HIP_OK(c, hipMemcpyAsync(c->adjc_bst[0], c->b, n2 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     if ((rc = ma_solve_dev(c, c->adjc_bst[0], mp, &d))) return rc;
     if (!d.converged)
-        return set_err(c, SSG_ERR_NOT_CONVERGED, "Monge-Ampere solve did not converge (rel. residual %.3e)",
-                       d.relative_residual);
+        return ma_not_converged(c, d);
     HIP_OK(c, hipMemcpyAsync(c->adjc_phis[0], c->ma_phi, c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice,
                              c->stream));
     if ((rc = flux_forward<double>(c, c->ncols_tend, c->adjc_bst[0]))) return rc;
@@ -1072,8 +1124,7 @@ int adjoint_coupled_sweep_ab2(ssg_ctx* c, const ssg_time_params* p, const ssg_ma
         HIP_OK(c, hipMemcpyAsync(c->adjc_bst[0], c->b, n2 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         if ((rc = ma_solve_dev(c, c->adjc_bst[0], mp, &d))) return rc;
         if (!d.converged)
-            return set_err(c, SSG_ERR_NOT_CONVERGED, "Monge-Ampere solve did not converge (rel. residual %.3e)",
-                           d.relative_residual);
+            return ma_not_converged(c, d);
         HIP_OK(c, hipMemcpyAsync(c->adjc_phis[0], c->ma_phi, c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice,
                                  c->stream));
         const bool euler = (n - 1) == 0;
@@ -1224,8 +1275,7 @@ int ssg_set_b(ssg_ctx* c, const double* host, int64_t n, const ssg_ma_params* p,
     if (d) *d = dd;
     if (rc) return rc;
     if (!dd.converged)
-        return set_err(c, SSG_ERR_NOT_CONVERGED, "Monge-Ampere solve did not converge (rel. residual %.3e)",
-                       dd.relative_residual);
+        return ma_not_converged(c, dd);
     return SSG_OK;
 }
 
@@ -1236,6 +1286,7 @@ int ssg_ma_solve(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
     HIP_OK(c, hipSetDevice(c->device));
     ssg_ma_diag local{};
     if (!d) d = &local;
+    c->ma_phi_current = false;
     HIP_OK(c, hipEventRecord(c->ev0, c->stream));
     if ((rc = ma_solve_dev(c, c->b, p, d))) return rc;
     HIP_OK(c, hipEventRecord(c->ev1, c->stream));
@@ -1245,8 +1296,7 @@ int ssg_ma_solve(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
     d->solve_ms = ms;
     c->last_diag = *d;
     if (!d->converged)
-        return set_err(c, SSG_ERR_NOT_CONVERGED, "not converged after %d iterations (rel. residual %.3e)", d->iterations,
-                       d->relative_residual);
+        return ma_not_converged(c, *d);
     return SSG_OK;
 }
 
@@ -1335,6 +1385,14 @@ int ssg_adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_param
     HIP_OK(c, hipMemcpyAsync(c->adj_lam, lam_in, bytes, hipMemcpyHostToDevice, c->stream));
     if (nsteps > 0 && (rc = adjoint_coupled(c, p, mp, step_start, nsteps, checkpoint_every))) return rc;
     ma_history_reset(c);  // the recomputation's solves do not continue a forward sequence
+    if (nsteps > 0) {
+        // b is back at the start state; φₛ and the velocities are those of
+        // the last recomputed stage: solve them for b again
+        ssg_ma_diag d{};
+        if ((rc = ma_solve_dev(c, c->b, mp, &d))) return rc;
+        if (!d.converged)
+            return set_err(c, SSG_ERR_NOT_CONVERGED, "Monge-Ampere solve for the start state did not converge");
+    }
     HIP_OK(c, hipMemcpyAsync(lam_out, c->adj_lam, bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     return SSG_OK;

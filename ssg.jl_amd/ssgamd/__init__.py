@@ -53,7 +53,7 @@ EXPORTED_SYMBOLS = (
     "ssg_last_ma_diag", "ssg_copy_probe", "ssg_adjoint", "ssg_ensemble_create", "ssg_ensemble_destroy",
     "ssg_ensemble_member", "ssg_ensemble_last_error", "ssg_ensemble_timestep", "ssg_ensemble_tuning",
     "ssg_ma_adjoint", "ssg_adjoint_coupled", "ssg_set_b", "ssg_cfl", "ssg_state_size", "ssg_save_state",
-    "ssg_load_state",
+    "ssg_load_state", "ssg_timestep_prepare",
 )
 
 MA_BODY = 0      # as wired: b on every level, surface slope 0
@@ -108,14 +108,14 @@ class _MaParams(ctypes.Structure):
 class _MaDiag(ctypes.Structure):
     _fields_ = [("converged", ctypes.c_int32), ("iterations", ctypes.c_int32), ("final_residual", ctypes.c_double),
                 ("relative_residual", ctypes.c_double), ("history", ctypes.c_double * 64),
-                ("solve_ms", ctypes.c_double), ("iterations_fp32", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("solve_ms", ctypes.c_double), ("iterations_fp32", ctypes.c_int32), ("not_elliptic", ctypes.c_int32)]
 
     def as_dict(self):
         n = min(self.iterations + 1, 64)
         return {"converged": bool(self.converged), "iterations": self.iterations,
                 "final_residual": self.final_residual, "relative_residual": self.relative_residual,
                 "convergence_history": list(self.history[:n]), "solve_time": self.solve_ms / 1e3,
-                "iterations_fp32": self.iterations_fp32}
+                "iterations_fp32": self.iterations_fp32, "not_elliptic": bool(self.not_elliptic)}
 
 
 _lib = None
@@ -140,6 +140,8 @@ def load_library(path: str | None = None):
     for n in ("ssg_set_field", "ssg_get_field", "ssg_set_field_device", "ssg_get_field_device"):
         getattr(lib, n).argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int64]
     lib.ssg_timestep.argtypes = [vp, ctypes.POINTER(_TimeParams), ctypes.POINTER(_TimeState), ctypes.c_int64]
+    lib.ssg_timestep_prepare.argtypes = [vp, ctypes.POINTER(_TimeParams), ctypes.POINTER(_TimeState),
+                                         ctypes.c_int64]
     lib.ssg_tendency.argtypes = [vp, _dp, ctypes.c_int32]
     lib.ssg_jacobian.argtypes = [vp, _dp]
     lib.ssg_rfft_2d.argtypes = [vp, _dp, _dp]
@@ -422,6 +424,15 @@ def timestep(fields: Fields, domain: Domain, params: TimeParams, state: TimeStat
         p = params._c()
     domain.check(domain.lib.ssg_timestep(domain.handle, ctypes.byref(p), ctypes.byref(state._c), int(nsteps)))
     return state.dt_actual
+
+
+def timestep_prepare(domain: Domain, params: TimeParams, state: TimeState, nsteps: int = 1) -> None:
+    """Launch set-up of a following ``timestep(..., nsteps)`` from ``state``:
+    autotuning and the step graphs it will replay, without stepping (no
+    reference counterpart)."""
+    p = params._c()
+    domain.check(domain.lib.ssg_timestep_prepare(domain.handle, ctypes.byref(p), ctypes.byref(state._c),
+                                                 int(nsteps)))
 
 
 def compute_tendency(fields: Fields, domain: Domain, ab2_reference_aliasing: bool = False) -> np.ndarray:

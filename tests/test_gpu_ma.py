@@ -401,3 +401,147 @@ def test_streaming_plane_access_is_bitwise_identical(gpu, prec, monkeypatch):
         d.close()
     assert np.array_equal(out["0"][0], out["1"][0])
     assert np.array_equal(out["0"][1], out["1"][1])
+
+
+def test_eps_closed_form_known_answer(gpu):
+    """ε = 0.1, body variant: Φ = A cos x + B cos y on every level solves the
+    residual operator for b = -A cos x - B cos y - εAB cos x cos y (its
+    Φxx Φyy - Φxy² is AB cos x cos y and Φzz = 0).  The GPU solve recovers Φ
+    on every level (horizontal mean 0, the gauge's value for this b)."""
+    n, nz, eps, A, B = 64, 8, 0.1, 0.8, -0.6
+    X, Y = grid(n, n)
+    b = -A * np.cos(X) - B * np.cos(Y) - eps * A * B * np.cos(X) * np.cos(Y)
+    d, f, diag = gpu_solve(gpu, b, nz, eps, gpu.MA_BODY, tol=1e-13, maxiter=80)
+    assert diag["converged"] and not diag["not_elliptic"]
+    expect = A * np.cos(X) + B * np.cos(Y)
+    phi = gpu.get_phi3d(d)
+    assert np.abs(phi - expect[None]).max() < 1e-11
+    assert np.abs(f.phis - expect).max() < 1e-11
+    d.close()
+
+
+def test_not_elliptic_is_rejected_up_front(gpu):
+    """Body variant with ε·max(b) >= 1: no elliptic solution exists
+    (det(I - εH) = 1 - εb at the z-independent fixed point), so the solve is
+    rejected before iterating, through ssg_ma_solve and through set_b!.  Just
+    inside the bound the same field solves normally."""
+    n, nz, eps = 64, 8, 0.3
+    b0 = smooth_b(n, 7, 6.0)
+    for scale, elliptic in ((1.05 / (eps * b0.max()), False), (0.5 / (eps * b0.max()), True)):
+        b = scale * b0
+        d = gpu.make_domain(n, n, nz)
+        f = gpu.allocate_fields(d)
+        f.bs = b
+        p = gpu._ma_params(eps, 1e-10, 60, gpu.MA_BODY, 0)
+        dg = gpu._MaDiag()
+        rc = d.lib.ssg_ma_solve(d.handle, ctypes.byref(p), ctypes.byref(dg))
+        if elliptic:
+            assert rc == 0 and dg.converged and not dg.not_elliptic
+        else:
+            assert rc == gpu.SSG_ERR_NOT_CONVERGED
+            assert dg.not_elliptic == 1 and dg.iterations == 0 and not dg.converged
+            assert b"not elliptic" in d.lib.ssg_last_error(d.handle)
+            diag = gpu.set_b(f, d, b, eps=eps, tol=1e-10, maxiter=60)
+            assert diag["not_elliptic"] and not diag["converged"]
+        d.close()
+
+
+def _coupled_run(gpu, b0, nz, eps, dt, scheme, steps, prec="SSG_FP64"):
+    nx = b0.shape[1]
+    ny = b0.shape[0]
+    d = gpu.make_domain(nx, ny, nz, precision=getattr(gpu, prec))
+    f = gpu.allocate_fields(d)
+    f.bs = b0
+    gpu.set_coupling(d, eps=eps, tol=1e-13, maxiter=80, variant=gpu.MA_BODY, warm=2)
+    st = gpu.TimeState(0.0, f)
+    gpu.timestep(f, d, gpu.TimeParams(dt, scheme=scheme, filter_freq=3, filter_strength=36.0, filter_order=8,
+                                      filter_cutoff=0.65), st, steps)
+    return d, f, st
+
+
+def _oracle_coupled(o, b0, eps, dt, scheme, steps, filter_freq=3):
+    from oracle.tangent import CoupledTangent
+    ct = CoupledTangent(o, eps, tol=1e-13)
+    b = b0.copy()
+    phi = None
+    fold = None
+    for step in range(1, steps + 1):
+        if scheme == 1:
+            ks = []
+            sb = b
+            for s in range(3):
+                phi = ct.solve(sb, phi)
+                ks.append(o.tendency(ct._psi(phi), sb))
+                sb = b + dt / 3 * ks[0] if s == 0 else b + 2 * dt / 3 * ks[1]
+            b = b + dt / 4 * (ks[0] + 3 * ks[2])
+        else:
+            phi = ct.solve(b, phi)
+            t = o.tendency(ct._psi(phi), b)
+            b = b + dt * t if step == 1 else b + dt * (1.5 * t - 0.5 * fold)
+            fold = t
+        if step % filter_freq == 0:
+            b = o.filter(b, 36.0, 8, 0.65)
+    phi = ct.solve(b, phi)
+    return b, ct._psi(phi)
+
+
+@pytest.mark.parametrize("scheme", [1, 0])
+def test_coupled_rectangular_against_oracle(gpu, scheme):
+    """Coupled RK3 and AB2 on a non-square 32×16×6 grid (multi-thread line
+    transforms along both axes), extrapolated warm starts (warm = 2) over
+    enough steps to reach the same-stage quadratic extrapolation, filter every
+    3 steps, against the oracle's step fed by Newton-Krylov solves of its
+    literal residual equations.  After the call φₛ belongs to the returned b:
+    it equals the oracle's solve for that b and a cold GPU solve of it."""
+    nx, ny, nz, eps, dt, steps = 32, 16, 6, 0.1, 5e-3, 6
+    b0 = spectral_b0_rect(nx, ny, 3, 2.0)
+    d, f, _ = _coupled_run(gpu, b0, nz, eps, dt, scheme, steps)
+    g, gphis = f.bs, f.phis
+    o = Oracle(nx, ny, nz)
+    b, psi = _oracle_coupled(o, b0, eps, dt, scheme, steps)
+    assert np.abs(g - b).max() < 1e-9 * np.abs(b).max()
+    assert np.abs((gphis - gphis.mean()) - psi).max() < 1e-9 * np.abs(psi).max()
+    # the same φₛ as a cold standalone solve of the returned b
+    d2, f2, diag = gpu_solve_rect(gpu, g, nz, eps)
+    assert diag["converged"]
+    assert np.abs(f2.phis - gphis).max() < 1e-10 * np.abs(gphis).max()
+    d.close()
+    d2.close()
+
+
+def spectral_b0_rect(nx, ny, seed, k0):
+    from ssgamd.ics import spectral_b0
+    return spectral_b0(nx, ny, seed=seed, k0=k0)
+
+
+def gpu_solve_rect(gpu, b, nz, eps):
+    d = gpu.make_domain(b.shape[1], b.shape[0], nz)
+    f = gpu.allocate_fields(d)
+    f.bs = b
+    diag = gpu.solve_monge_ampere(f, d, eps=eps, tol=1e-13, maxiter=80, variant=gpu.MA_BODY)
+    return d, f, diag
+
+
+@pytest.mark.parametrize("scheme", [1, 0])
+@pytest.mark.parametrize("prec", ["SSG_FP64", "SSG_FP32_MIXED"])
+def test_coupled_outputs_follow_returned_b(gpu, scheme, prec):
+    """After coupled steps φₛ, the velocities (CFL) and the surface kinetic
+    energy belong to the returned b, in both precisions: they equal those of
+    a separate context that solves the returned b cold.  A second call
+    continues from the solve that ended the first."""
+    n, nz, eps, dt = 128, 8, 0.1, 1e-3
+    b0 = smooth_b(n, 4, 6.0)
+    d, f, st = _coupled_run(gpu, b0, nz, eps, dt, scheme, 4, prec)
+    gpu.timestep(f, d, gpu.TimeParams(dt, scheme=scheme, filter_freq=3, filter_strength=36.0, filter_order=8,
+                                      filter_cutoff=0.65), st, 3)
+    b = f.bs
+    d2, f2, diag = gpu_solve_rect(gpu, b, nz, eps)
+    assert diag["converged"]
+    tol = 1e-9
+    assert np.abs(f.phis - f2.phis).max() < tol * np.abs(f2.phis).max()
+    k1, k2 = gpu.diagnostics(d)["ke_surface"], gpu.diagnostics(d2)["ke_surface"]
+    assert abs(k1 - k2) < tol * k2
+    c1, c2 = gpu.compute_cfl_number(f, d, dt), gpu.compute_cfl_number(f2, d2, dt)
+    assert abs(c1 - c2) < tol * c2
+    d.close()
+    d2.close()

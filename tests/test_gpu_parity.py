@@ -260,8 +260,10 @@ def test_full_size_coupled_properties(gpu):
 
 
 def test_graph_replay_matches_eager(gpu, monkeypatch):
-    """The captured filter-cycle graph, replayed many times with eager work and
-    synchronisations in between, gives bitwise the eager launches' result."""
+    """The step graphs (whole filter cycles and the partial cycles at both ends
+    of a 23-step call from every phase), replayed many times with eager work
+    and synchronisations in between, some captured ahead of the call by
+    ssg_timestep_prepare, give bitwise the eager launches' result."""
     from ssgamd.ics import spectral_b0, streamfunction_from_b
     nx = ny = 256
     b0 = spectral_b0(nx, ny, seed=3)
@@ -278,7 +280,9 @@ def test_graph_replay_matches_eager(gpu, monkeypatch):
         f.bs = b0
         st = gpu.TimeState(0.0, f)
         snaps = []
-        for _ in range(4):
+        for i in range(4):
+            if mode == "1" and i % 2:
+                gpu.timestep_prepare(d, params, st, 23)   # graphs captured ahead of the call
             gpu.timestep(f, d, params, st, 23)
             snaps.append(f.bs)
             gpu.diagnostics(d)           # eager work between replays

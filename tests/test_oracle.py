@@ -204,6 +204,56 @@ def test_k12_ma_residual_zero_at_poisson_solution():
     assert np.abs(phi_bc - phi).max() < 1e-14
 
 
+def test_ma_residual_eps_closed_form():
+    """ε > 0, b on every level, g = 0 (src/poisson.jl:228-253): Φ = A cos x +
+    B cos y on every level has Φxx Φyy - Φxy² = AB cos x cos y and Φzz = 0,
+    so b = -A cos x - B cos y - εAB cos x cos y makes the residual vanish.
+    The same Φ and b at -ε leave r = 2εAB cos x cos y on interior levels,
+    which pins the sign of the ε term."""
+    nx, ny, nz, eps, A, B = 16, 16, 6, 0.1, 0.8, -0.6
+    X, Y = grid(nx, ny)
+    o = Oracle(nx, ny, nz)
+    phi2 = A * np.cos(X) + B * np.cos(Y)
+    det = A * B * np.cos(X) * np.cos(Y)
+    b2 = -A * np.cos(X) - B * np.cos(Y) - eps * det
+    phi = np.stack([phi2] * nz)
+    b3 = np.stack([b2] * nz)
+    assert np.abs(o.ma_d_operator(phi) - det[None]).max() < 1e-13
+    r, phi_bc = o.ma_residual(phi, b3, None, eps)
+    assert np.abs(r).max() < 1e-13
+    assert np.abs(phi_bc - phi).max() < 1e-14
+    r_flip, _ = o.ma_residual(phi, b3, None, -eps)
+    assert np.abs(r_flip[1:-1] - 2 * eps * det[None]).max() < 1e-13
+    assert np.abs(r_flip[0]).max() == 0 and np.abs(r_flip[-1]).max() == 0   # BC rows zero r
+
+
+def test_stiff_configuration_is_not_elliptic():
+    """The stiff configuration (2048², seed 7, spectrum peak k0 = 12, rms 1,
+    ε = 0.3): the body-forced problem's z-independent fixed point has
+    det(I - εH) = 1 - εb, so an elliptic solution needs ε·max(b) < 1.  Its
+    b0 exceeds 1/ε at a few hundred points, so no elliptic solution exists
+    (the solver rejects such a solve up front)."""
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ssg.jl_amd"))
+    from ssgamd.ics import spectral_b0
+    b = spectral_b0(2048, 2048, seed=7, k0=12.0)
+    assert abs(np.sqrt(np.mean(b * b)) - 1.0) < 1e-12
+    assert 0.3 * b.max() > 1.04
+    assert 300 < int((0.3 * b >= 1.0).sum()) < 400
+    # the identity itself, on a smooth field: det(I - εH) = 1 - ε(tr H - ε det H)
+    nx = 64
+    X, Y = grid(nx, nx)
+    phi = 0.7 * np.cos(X) * np.sin(2 * Y) + 0.2 * np.sin(3 * X)
+    hxx = -0.7 * np.cos(X) * np.sin(2 * Y) - 1.8 * np.sin(3 * X)
+    hyy = -2.8 * np.cos(X) * np.sin(2 * Y)
+    hxy = -1.4 * np.sin(X) * np.cos(2 * Y)
+    eps = 0.3
+    lhs = (1 - eps * hxx) * (1 - eps * hyy) - (eps * hxy) ** 2
+    bb = (hxx + hyy) - eps * (hxx * hyy - hxy ** 2)
+    assert np.abs(lhs - (1 - eps * bb)).max() < 1e-13
+    assert phi.shape == (nx, nx)
+
+
 def test_ma_boundary_rows_one_sided_neumann():
     """BC rows (src/poisson.jl:309-364): a quadratic in z with slope g at the top and
     0 at the bottom satisfies both one-sided formulas exactly."""
