@@ -131,6 +131,8 @@ typedef struct {
    src/fields.jl:72-116, src/timestep.jl:45-86) */
 int ssg_create(ssg_ctx** out, const ssg_domain_desc* desc);
 int ssg_destroy(ssg_ctx* ctx);
+/* ctx = NULL gives the message of this thread's last failed context-free
+   call (ssg_tendency_noalloc) */
 const char* ssg_last_error(const ssg_ctx* ctx);
 int ssg_sync(ssg_ctx* ctx);
 
@@ -160,6 +162,21 @@ int ssg_timestep_prepare(ssg_ctx* ctx, const ssg_time_params* params, const ssg_
    out (nx*ny).  ab2_reference_aliasing = 1 gives the tendency the AB2 call
    site produces (src/timestep.jl:159). */
 int ssg_tendency(ssg_ctx* ctx, double* out, int32_t ab2_reference_aliasing);
+/* Allocation-free variant of ssg_tendency.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under stream or graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  It needs no context: b, phi and out are device pointers to
+   nx*ny FP64 fields (x fastest), desc gives the grid (FP64 only; nz is
+   ignored) and stream the HIP stream (NULL: the default stream).  The
+   kernels are enqueued and the call returns without synchronising; it calls
+   no hipMalloc/hipFree.  All intermediates live in one file-scope device
+   array of 40 MiB per GPU (held until the process ends), which fits FP64
+   grids up to 1024 x 1024 points; larger grids return SSG_ERR_UNSUPPORTED.
+   Every call on a GPU shares that array, so calls that overlap on one GPU
+   (other streams, other host threads) race; the array's contents are
+   rewritten by each call before they are read.  Errors: ssg_last_error(NULL). */
+int ssg_tendency_noalloc(const ssg_domain_desc* desc, const double* b, const double* phi, double* out,
+                         int32_t ab2_reference_aliasing, void* stream);
 /* compute_jacobian! (src/nonlinear.jl:42-57): -J(φₛ, b), not dealiased */
 int ssg_jacobian(ssg_ctx* ctx, double* out);
 

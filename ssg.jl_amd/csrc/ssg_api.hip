@@ -505,7 +505,9 @@ int ssg_destroy(ssg_ctx* c) {
     return SSG_OK;
 }
 
-const char* ssg_last_error(const ssg_ctx* c) { return c ? c->err : "null context"; }
+// With a null context: the message of this thread's last failed
+// context-free call (ssg_tendency_noalloc), or "" if it succeeded.
+const char* ssg_last_error(const ssg_ctx* c) { return c ? c->err : noalloc_last_error(); }
 
 int ssg_sync(ssg_ctx* c) {
     if (!c) return SSG_ERR_ARG;

@@ -542,6 +542,7 @@ inline int ma_not_converged(ssg_ctx* c, const ssg_ma_diag& d) {
 
 // shared across translation units
 extern std::mutex g_tune_mu;
+const char* noalloc_last_error();
 int ma_solve_dev(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_diag* d);
 int rfft2_device(ssg_ctx* c, const double* src, cpx<double>* dst = nullptr);
 int ma_surface_outputs(ssg_ctx* c);

@@ -53,7 +53,7 @@ EXPORTED_SYMBOLS = (
     "ssg_last_ma_diag", "ssg_copy_probe", "ssg_adjoint", "ssg_ensemble_create", "ssg_ensemble_destroy",
     "ssg_ensemble_member", "ssg_ensemble_last_error", "ssg_ensemble_timestep", "ssg_ensemble_tuning",
     "ssg_ma_adjoint", "ssg_adjoint_coupled", "ssg_set_b", "ssg_cfl", "ssg_state_size", "ssg_save_state",
-    "ssg_load_state", "ssg_timestep_prepare",
+    "ssg_load_state", "ssg_timestep_prepare", "ssg_tendency_noalloc",
 )
 
 MA_BODY = 0      # as wired: b on every level, surface slope 0
@@ -143,6 +143,8 @@ def load_library(path: str | None = None):
     lib.ssg_timestep_prepare.argtypes = [vp, ctypes.POINTER(_TimeParams), ctypes.POINTER(_TimeState),
                                          ctypes.c_int64]
     lib.ssg_tendency.argtypes = [vp, _dp, ctypes.c_int32]
+    lib.ssg_tendency_noalloc.argtypes = [ctypes.POINTER(_DomainDesc), ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
     lib.ssg_jacobian.argtypes = [vp, _dp]
     lib.ssg_rfft_2d.argtypes = [vp, _dp, _dp]
     lib.ssg_irfft_2d.argtypes = [vp, _dp, _dp]
@@ -440,6 +442,20 @@ def compute_tendency(fields: Fields, domain: Domain, ab2_reference_aliasing: boo
     out = np.empty((domain.Ny, domain.Nx))
     domain.check(domain.lib.ssg_tendency(domain.handle, out.ctypes.data_as(_dp), int(ab2_reference_aliasing)))
     return out
+
+
+def compute_tendency_noalloc(b_ptr: int, phi_ptr: int, out_ptr: int, Nx: int, Ny: int, *, Lx=2 * np.pi,
+                             Ly=2 * np.pi, ab2_reference_aliasing: bool = False, stream: int = 0,
+                             device: int = 0) -> None:
+    """``compute_tendency!`` of device-resident fields without any device
+    allocation (``ssg_tendency_noalloc``): ``b_ptr``, ``phi_ptr`` and
+    ``out_ptr`` are device addresses of Nx*Ny FP64 arrays (x fastest); the
+    kernels are enqueued on ``stream`` and the call does not synchronise."""
+    lib = load_library()
+    desc = _DomainDesc(int(Nx), int(Ny), 1, float(Lx), float(Ly), 1.0, SSG_FP64, int(device))
+    rc = lib.ssg_tendency_noalloc(ctypes.byref(desc), b_ptr, phi_ptr, out_ptr, int(ab2_reference_aliasing), stream)
+    if rc != SSG_OK:
+        raise SSGError(rc, lib.ssg_last_error(None).decode(errors="replace"))
 
 
 def compute_jacobian(fields: Fields, domain: Domain) -> np.ndarray:
