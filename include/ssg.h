@@ -91,8 +91,11 @@ typedef struct {
     double sum_b;          /* Σ b */
     double sum_b2;         /* Σ b² */
     double parseval_b;     /* parsevalsum2(rfft2(b)), src/transforms.jl:230-274 */
-    double ke_surface;     /* compute_surface_kinetic_energy, src/setICs.jl:225-246 */
+    double ke_surface;     /* compute_surface_kinetic_energy, src/setICs.jl:225-246: the reference's
+                              formula, parsevalsum2 weights along x and the factor Lz included */
     int64_t nonfinite;     /* count of non-finite b entries */
+    double ke_true;        /* ½ ∫∫ (u² + v²) dx dy of the surface velocities, the energy the reference's
+                              formula stands for (no Lz, every Fourier mode counted once) */
 } ssg_diag;
 
 /* Monge-Ampère problem variants (src/poisson.jl:1068-1131):
@@ -201,6 +204,15 @@ int ssg_diagnostics(ssg_ctx* ctx, ssg_diag* out);
    the iteration diverges, and, without iterating, when a cold solve of the
    body variant has eps*max(b) >= 1 (diag->not_elliptic = 1). */
 int ssg_ma_solve(ssg_ctx* ctx, const ssg_ma_params* params, ssg_ma_diag* diag);
+/* Vertical grid of the Monge-Ampère problem: the z (nz levels, strictly
+   increasing, surface last) and dz (nz spacings) a Domain carries
+   (src/domain.jl:238-281).  dz = NULL applies make_vertical_grid's :custom
+   rule (src/domain.jl:258-275: dz[1] = z[2]-z[1], dz[i] = (z[i+1]-z[i-1])/2,
+   dz[Nz] = z[Nz]-z[Nz-1]).  The vertical operator uses dz exactly as
+   d2dz2! (src/transforms.jl:798-850) and the one-sided Neumann rows
+   (src/poisson.jl:309-364) do.  Without this call the grid is uniform,
+   dz = Lz/(Nz-1) (src/domain.jl:238-249). */
+int ssg_set_vertical_grid(ssg_ctx* ctx, const double* z, const double* dz, int32_t nz);
 /* fields.φ (nx*ny*nz, x fastest then y then z) after ssg_ma_solve */
 int ssg_get_phi3d(ssg_ctx* ctx, double* host, int64_t n);
 /* Coupled stepping: with params != NULL, ssg_timestep re-solves the

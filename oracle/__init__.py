@@ -18,7 +18,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libssg_oracle.so")
+# SSG_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of `make -C oracle sanitize`, used by oracle/run_sanitized_tests.sh)
+_LIB_PATH = os.environ.get("SSG_ORACLE_LIB") or os.path.join(_HERE, "libssg_oracle.so")
 _lib = None
 
 _dp = ctypes.POINTER(ctypes.c_double)

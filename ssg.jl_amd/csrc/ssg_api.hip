@@ -628,7 +628,14 @@ int ssg_timestep(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, int64_
     int rc = c->precision == SSG_FP32_MIXED ? run_steps<float>(c, p, s, nsteps) : run_steps<double>(c, p, s, nsteps);
     if (rc) return rc;
     HIP_OK(c, hipEventRecord(c->ev1, c->stream));
-    HIP_OK(c, hipStreamSynchronize(c->stream));
+    // wait by polling the end event: a blocking stream synchronisation adds
+    // its wake-up latency to every call, which shows on short step runs
+    for (;;) {
+        const hipError_t q = hipEventQuery(c->ev1);
+        if (q == hipSuccess) break;
+        if (q != hipErrorNotReady) return set_err(c, SSG_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+        __builtin_ia32_pause();
+    }
     float ms = 0;
     HIP_OK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
@@ -940,6 +947,15 @@ int ssg_diagnostics(ssg_ctx* c, ssg_diag* out) {
     const double kv = parseval_of_tp(c, &rc);
     if (rc) return rc;
     out->ke_surface = 0.5 * ku + 0.5 * kv;
+    // ½ ∫∫ |w|² dx dy in physical space (fixed-order host sum of the block partials)
+    const int nb = c->npartial;
+    hipLaunchKernelGGL(k_w_energy, dim3(nb), dim3(256), 0, c->stream, (const double2*)c->d.w, n, c->partial);
+    HIP_OK(c, hipGetLastError());
+    HIP_OK(c, hipMemcpyAsync(c->partial_host, c->partial, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    double e2 = 0;
+    for (int i = 0; i < nb; ++i) e2 += c->partial_host[i];
+    out->ke_true = 0.5 * e2 * (c->lx * c->ly) / ((double)c->nx * c->ny);
     return SSG_OK;
 }
 

@@ -105,6 +105,7 @@ struct ssg_ctx {
     cpx<float>* ma32_d = nullptr;
     int ma_ept[2] = {0, 0};           // elements per thread of k_ma_rows per path (0 FP64, 1 FP32)
     Grid g3{};                        // 3D plane geometry: pitch = nz * ny
+    std::vector<double> dz;           // vertical spacings (Nz entries) of a custom grid; empty: uniform Lz/(Nz-1)
     // coupled stepping (MA solve at every stage)
     bool coupled = false;
     ssg_ma_params ma_par{};

@@ -940,6 +940,16 @@ __global__ void k_convert(const A* __restrict__ in, B* __restrict__ out, long n)
         out[q] = (B)in[q];
 }
 
+// per-block Σ (u² + v²) of the interleaved velocities
+static __global__ __launch_bounds__(256) void k_w_energy(const double2* __restrict__ w, long n, double* partial) {
+    double v[1] = {0.0};
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const double2 z = w[q];
+        v[0] += z.x * z.x + z.y * z.y;
+    }
+    block_sum<1>(v, partial);
+}
+
 // per-block max of b (-inf for a block with no points; NaN propagates)
 static __global__ __launch_bounds__(256) void k_bmax(const double* __restrict__ b, long n, double* partial) {
     __shared__ double red[256];

@@ -18,6 +18,44 @@ void hist_free(SolveHistory& h);
 // overrides the choice; both give bitwise the same results.
 constexpr size_t kMaStreamBytes = 128u << 20;
 
+// Stencil coefficients of the vertical operator for the context's dz (the
+// uniform grid's Lz/(Nz-1) unless ssg_set_vertical_grid gave one): per
+// interior level k the d2dz2! weights (α, β, γ) with h₋ = dz[k-1],
+// h₊ = dz[k] (src/transforms.jl:798-850), then the one-sided Neumann rows
+// of the bottom and the surface (src/poisson.jl:309-364).
+std::vector<double> ma_coefficients(const ssg_ctx* c) {
+    const int nz = c->nz;
+    std::vector<double> dz = c->dz;
+    if ((int)dz.size() != nz) dz.assign(nz, c->lz / (nz - 1));
+    std::vector<double> cf(3 * nz + 6, 0.0);
+    for (int k = 1; k <= nz - 2; ++k) {
+        const double dzm = dz[k - 1], dzp = dz[k];
+        cf[3 * k] = 2.0 / (dzm * (dzm + dzp));
+        cf[3 * k + 1] = -2.0 / (dzm * dzp);
+        cf[3 * k + 2] = 2.0 / (dzp * (dzm + dzp));
+    }
+    {  // apply_neumann_bottom!, src/poisson.jl:343-364
+        const double h1 = dz[0], h2 = dz[1], al = h1 + h2;
+        cf[3 * nz] = -(2 * h1 + h2) / (h1 * al);
+        cf[3 * nz + 1] = al / (h1 * h2);
+        cf[3 * nz + 2] = -h1 / (h2 * al);
+    }
+    {  // apply_neumann_surface!, src/poisson.jl:309-336
+        const double h1 = dz[nz - 2], h2 = dz[nz - 3], al = h1 + h2;
+        cf[3 * nz + 3] = (2 * h1 + h2) / (h1 * al);
+        cf[3 * nz + 4] = -al / (h1 * h2);
+        cf[3 * nz + 5] = h1 / (h2 * al);
+    }
+    return cf;
+}
+
+int ma_upload_coef(ssg_ctx* c) {
+    const std::vector<double> cf = ma_coefficients(c);
+    HIP_OK(c, hipMemcpyAsync(c->ma_coef, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
 int ma_alloc(ssg_ctx* c) {
     if (c->ma_ready) return SSG_OK;
     if (c->nz < 3 || c->nz > 128)
@@ -45,30 +83,9 @@ int ma_alloc(ssg_ctx* c) {
     c->ma_sum_host->r2 = c->ma_sum_host->h2 = kMaSumUnset;
     HIP_OK(c, hipHostGetDevicePointer((void**)&c->ma_sum_dev, c->ma_sum_host, 0));
     HIP_OK(c, hipMalloc(&c->bst, (size_t)c->nx * c->ny * sizeof(double)));
-    // stencil coefficients (uniform grid dz of the context)
-    const int nz = c->nz;
-    std::vector<double> dz(nz, c->lz / (nz - 1));
-    std::vector<double> cf(3 * nz + 6, 0.0);
-    for (int k = 1; k <= nz - 2; ++k) {  // d2dz2!, src/transforms.jl:812-819
-        const double dzm = dz[k - 1], dzp = dz[k];
-        cf[3 * k] = 2.0 / (dzm * (dzm + dzp));
-        cf[3 * k + 1] = -2.0 / (dzm * dzp);
-        cf[3 * k + 2] = 2.0 / (dzp * (dzm + dzp));
-    }
-    {  // apply_neumann_bottom!, src/poisson.jl:343-364
-        const double h1 = dz[0], h2 = dz[1], al = h1 + h2;
-        cf[3 * nz] = -(2 * h1 + h2) / (h1 * al);
-        cf[3 * nz + 1] = al / (h1 * h2);
-        cf[3 * nz + 2] = -h1 / (h2 * al);
-    }
-    {  // apply_neumann_surface!, src/poisson.jl:309-336
-        const double h1 = dz[nz - 2], h2 = dz[nz - 3], al = h1 + h2;
-        cf[3 * nz + 3] = (2 * h1 + h2) / (h1 * al);
-        cf[3 * nz + 4] = -al / (h1 * h2);
-        cf[3 * nz + 5] = h1 / (h2 * al);
-    }
-    HIP_OK(c, hipMalloc(&c->ma_coef, cf.size() * sizeof(double)));
-    HIP_OK(c, hipMemcpy(c->ma_coef, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(c, hipMalloc(&c->ma_coef, (3 * (size_t)c->nz + 6) * sizeof(double)));
+    int rc = ma_upload_coef(c);
+    if (rc) return rc;
     c->g3 = c->g;
     c->g3.pitch = c->nz * c->ny;
     c->ma_ready = true;
@@ -1321,6 +1338,37 @@ int ssg_get_phi3d(ssg_ctx* c, double* host, int64_t n) {
     if ((rc = launch_rows_inv<double, MODE_PAIR, EPI_STORE>(c, r, c->nz * c->ny, &c->g3))) return rc;
     HIP_OK(c, hipMemcpyAsync(host, c->ma_p[1], (size_t)need * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
+
+int ssg_set_vertical_grid(ssg_ctx* c, const double* z, const double* dz, int32_t nz) {
+    if (!c) return SSG_ERR_ARG;
+    if (!z) return set_err(c, SSG_ERR_ARG, "null z");
+    if (nz != c->nz) return set_err(c, SSG_ERR_ARG, "Custom z_coords length (%d) must match Nz (%d)", nz, c->nz);
+    if (nz < 3) return set_err(c, SSG_ERR_UNSUPPORTED, "a vertical grid needs nz >= 3 (nz = %d)", nz);
+    for (int k = 0; k + 1 < nz; ++k)
+        if (!(z[k + 1] > z[k]) || !std::isfinite(z[k]) || !std::isfinite(z[k + 1]))
+            return set_err(c, SSG_ERR_ARG, "z must be finite and strictly increasing (z[%d] = %g, z[%d] = %g)", k, z[k],
+                           k + 1, z[k + 1]);
+    std::vector<double> h(nz);
+    if (dz) {
+        for (int k = 0; k < nz; ++k) {
+            if (!(dz[k] > 0) || !std::isfinite(dz[k]))
+                return set_err(c, SSG_ERR_ARG, "dz[%d] = %g must be finite and > 0", k, dz[k]);
+            h[k] = dz[k];
+        }
+    } else {
+        // the :custom rule of make_vertical_grid (src/domain.jl:258-275)
+        for (int k = 1; k + 1 < nz; ++k) h[k] = 0.5 * (z[k + 1] - z[k - 1]);
+        h[0] = z[1] - z[0];
+        h[nz - 1] = z[nz - 1] - z[nz - 2];
+    }
+    HIP_OK(c, hipSetDevice(c->device));
+    c->dz = h;
+    // the operator changed: no earlier solve continues into this one
+    ma_history_reset(c);
+    c->ma_have_d = false;
+    if (c->ma_ready) return ma_upload_coef(c);
     return SSG_OK;
 }
 

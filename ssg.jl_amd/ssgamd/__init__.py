@@ -53,7 +53,7 @@ EXPORTED_SYMBOLS = (
     "ssg_last_ma_diag", "ssg_copy_probe", "ssg_adjoint", "ssg_ensemble_create", "ssg_ensemble_destroy",
     "ssg_ensemble_member", "ssg_ensemble_last_error", "ssg_ensemble_timestep", "ssg_ensemble_tuning",
     "ssg_ma_adjoint", "ssg_adjoint_coupled", "ssg_set_b", "ssg_cfl", "ssg_state_size", "ssg_save_state",
-    "ssg_load_state", "ssg_timestep_prepare", "ssg_tendency_noalloc",
+    "ssg_load_state", "ssg_timestep_prepare", "ssg_tendency_noalloc", "ssg_set_vertical_grid",
 )
 
 MA_BODY = 0      # as wired: b on every level, surface slope 0
@@ -97,7 +97,7 @@ class _TimeState(ctypes.Structure):
 
 class _Diag(ctypes.Structure):
     _fields_ = [("sum_b", ctypes.c_double), ("sum_b2", ctypes.c_double), ("parseval_b", ctypes.c_double),
-                ("ke_surface", ctypes.c_double), ("nonfinite", ctypes.c_int64)]
+                ("ke_surface", ctypes.c_double), ("nonfinite", ctypes.c_int64), ("ke_true", ctypes.c_double)]
 
 
 class _MaParams(ctypes.Structure):
@@ -143,6 +143,7 @@ def load_library(path: str | None = None):
     lib.ssg_timestep_prepare.argtypes = [vp, ctypes.POINTER(_TimeParams), ctypes.POINTER(_TimeState),
                                          ctypes.c_int64]
     lib.ssg_tendency.argtypes = [vp, _dp, ctypes.c_int32]
+    lib.ssg_set_vertical_grid.argtypes = [vp, _dp, _dp, ctypes.c_int32]
     lib.ssg_tendency_noalloc.argtypes = [ctypes.POINTER(_DomainDesc), ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
     lib.ssg_jacobian.argtypes = [vp, _dp]
@@ -491,7 +492,7 @@ def diagnostics(domain: Domain) -> dict:
     d = _Diag()
     domain.check(domain.lib.ssg_diagnostics(domain.handle, ctypes.byref(d)))
     return {"sum_b": d.sum_b, "sum_b2": d.sum_b2, "parseval_b": d.parseval_b, "ke_surface": d.ke_surface,
-            "nonfinite": d.nonfinite}
+            "nonfinite": d.nonfinite, "ke_true": d.ke_true}
 
 
 def _ma_params(eps, tol, maxiter, variant, warm):
@@ -514,6 +515,41 @@ def solve_monge_ampere(fields: Fields, domain: Domain, *, eps: float = 0.1, tol:
     out = d.as_dict()
     out["epsilon_parameter"] = float(eps)
     return out
+
+
+def make_vertical_grid(Nz: int, Lz: float = 1.0, z_grid: str = "uniform", z_coords=None):
+    """``make_vertical_grid`` (src/domain.jl:238-281) for the non-periodic
+    grids: ``uniform`` (z = range(0, Lz, Nz), dz = Lz/(Nz-1) with the last
+    entry duplicated) and ``custom`` (z = z_coords; dz centred inside, one-sided
+    at both ends).  Returns (z, dz)."""
+    if z_grid == "uniform":
+        z = np.linspace(0.0, Lz, Nz)
+        dz = np.full(Nz, Lz / (Nz - 1))
+        return z, dz
+    if z_grid == "custom":
+        if z_coords is None:
+            raise ValueError("For :custom grid, provide z_coords")
+        z = np.array(z_coords, dtype=np.float64)
+        if z.size != Nz:
+            raise ValueError(f"Custom z_coords length ({z.size}) must match Nz ({Nz})")
+        dz = np.zeros(Nz)
+        dz[1:-1] = 0.5 * (z[2:] - z[:-2])
+        dz[0] = z[1] - z[0]
+        dz[-1] = z[-1] - z[-2]
+        return z, dz
+    raise ValueError(f"Unknown z_grid type: {z_grid}. Use uniform or custom")
+
+
+def set_vertical_grid(domain: Domain, z, dz=None) -> None:
+    """The Monge-Ampère problem's vertical grid (``ssg_set_vertical_grid``):
+    levels z and spacings dz as a Domain holds them (dz None: the :custom
+    rule applied to z)."""
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    dzp = None
+    if dz is not None:
+        dz = np.ascontiguousarray(dz, dtype=np.float64)
+        dzp = dz.ctypes.data_as(_dp)
+    domain.check(domain.lib.ssg_set_vertical_grid(domain.handle, z.ctypes.data_as(_dp), dzp, int(z.size)))
 
 
 def get_phi3d(domain: Domain) -> np.ndarray:

@@ -545,3 +545,68 @@ def test_coupled_outputs_follow_returned_b(gpu, scheme, prec):
     assert abs(c1 - c2) < tol * c2
     d.close()
     d2.close()
+
+
+def _stretched_z(nz):
+    """Levels on [0, 1], finer toward the surface (tanh stretching)."""
+    s = np.linspace(0.0, 1.0, nz)
+    return 1.0 + np.tanh(2.0 * (s - 1.0)) / np.tanh(2.0)
+
+
+@pytest.mark.parametrize("rule", ["true_spacing", "custom"])
+def test_custom_vertical_grid_surface_newton(gpu, rule):
+    """The surface-forced problem (where the vertical grid matters) on a
+    stretched grid: ε = 0 at 16²×6 and ε = 0.1 at 8²×5, against the dense
+    Newton solve of the oracle's residual equations on the same z, dz (the
+    spacings given explicitly, or dz from make_vertical_grid's :custom
+    rule)."""
+    for n, nz, eps, amp in ((16, 6, 0.0, 1.0), (8, 5, 0.1, 0.2)):
+        z = _stretched_z(nz)
+        if rule == "custom":
+            _, dz = gpu.make_vertical_grid(nz, 1.0, "custom", z)
+        else:
+            dz = np.append(np.diff(z), z[-1] - z[-2])
+        b = smooth_b(n, 4, 1.5) * amp
+        d = gpu.make_domain(n, n, nz)
+        f = gpu.allocate_fields(d)
+        gpu.set_vertical_grid(d, z, None if rule == "custom" else dz)
+        f.bs = b
+        diag = gpu.solve_monge_ampere(f, d, eps=eps, tol=1e-13, maxiter=100, variant=gpu.MA_SURFACE)
+        assert diag["converged"], diag
+        phi = gpu.get_phi3d(d)
+        d.close()
+        o = Oracle(n, n, nz)
+        o.set_vertical_grid(z, dz)
+        ref, _ = ma_newton(o, np.zeros((nz, n, n)), b, eps)
+        assert np.abs(demean(phi) - demean(ref)).max() < 1e-9 * np.abs(ref).max(), (n, nz, eps)
+        r, _ = o.ma_residual(phi, np.zeros((nz, n, n)), b, eps)
+        assert np.abs(r).max() < 1e-9
+
+
+def test_vertical_grid_changes_the_solution_and_rejects(gpu):
+    """A stretched grid gives a different surface solution from the uniform
+    one (the spacings reach the operator); bad grids are rejected."""
+    n, nz = 16, 6
+    b = smooth_b(n, 4, 1.5)
+    d = gpu.make_domain(n, n, nz)
+    f = gpu.allocate_fields(d)
+    f.bs = b
+    gpu.solve_monge_ampere(f, d, eps=0.0, tol=1e-13, variant=gpu.MA_SURFACE)
+    uni = f.phis
+    gpu.set_vertical_grid(d, _stretched_z(nz))
+    gpu.solve_monge_ampere(f, d, eps=0.0, tol=1e-13, variant=gpu.MA_SURFACE)
+    assert np.abs(f.phis - uni).max() > 1e-3 * np.abs(uni).max()
+    z_uni, dz_uni = gpu.make_vertical_grid(nz, 1.0)
+    gpu.set_vertical_grid(d, z_uni, dz_uni)
+    gpu.solve_monge_ampere(f, d, eps=0.0, tol=1e-13, variant=gpu.MA_SURFACE)
+    assert np.array_equal(f.phis, uni)
+    lib = d.lib
+    z = _stretched_z(nz)
+    bad = z[::-1].copy()
+    with pytest.raises(gpu.SSGError):
+        gpu.set_vertical_grid(d, bad)
+    with pytest.raises(gpu.SSGError):
+        gpu.set_vertical_grid(d, z[:-1])
+    with pytest.raises(gpu.SSGError):
+        gpu.set_vertical_grid(d, z, -np.ones(nz))
+    d.close()

@@ -177,6 +177,15 @@ def test_diagnostics(gpu):
     ke = 0.5 * o.parsevalsum2(o.rfft2(u)) + 0.5 * o.parsevalsum2(o.rfft2(v))
     assert abs(dg["ke_surface"] - ke) < 1e-10 * ke
     assert dg["nonfinite"] == 0
+    area = (2 * np.pi) ** 2 / (nx * ny)
+    assert abs(dg["ke_true"] - 0.5 * np.sum(u * u + v * v) * area) < 1e-12 * dg["ke_true"]
+    # closed form: ψ = sin x cos 2y gives u = 2 sin x sin 2y, v = cos x cos 2y and
+    # ½∫∫(u² + v²) = ½ (4π² + π²); the reference's formula counts x-modes 1..Nx/2-1
+    # twice and multiplies by Lz
+    X, Y = np.meshgrid(2 * np.pi / nx * np.arange(nx), 2 * np.pi / ny * np.arange(ny))
+    f.phis = np.sin(X) * np.cos(2 * Y)
+    dg = gpu.diagnostics(d)
+    assert abs(dg["ke_true"] - 2.5 * np.pi ** 2) < 1e-12 * dg["ke_true"]
 
 
 def test_error_paths(gpu):

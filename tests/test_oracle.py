@@ -270,6 +270,36 @@ def test_ma_boundary_rows_one_sided_neumann():
     assert np.abs(out - phi).max() < 1e-14
 
 
+def test_vertical_grid_rules_and_nonuniform_rows():
+    """make_vertical_grid (src/domain.jl:238-281): uniform dz = Lz/(Nz-1)
+    with the last entry duplicated; :custom dz centred inside and one-sided
+    at the ends.  On a non-uniform grid given its true spacings, the
+    d2dz2! stencil and both one-sided Neumann rows are exact for a quadratic
+    profile: Φ = a z² has Φzz = 2a and slopes 0 at z = 0, 2a at z = 1."""
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ssg.jl_amd"))
+    from ssgamd import make_vertical_grid
+    z, dz = make_vertical_grid(5, 2.0)
+    assert np.allclose(z, [0, 0.5, 1.0, 1.5, 2.0]) and np.allclose(dz, 0.5)
+    z, dz = make_vertical_grid(5, 1.0, "custom", [0.0, 0.1, 0.3, 0.6, 1.0])
+    assert np.allclose(dz, [0.1, 0.15, 0.25, 0.35, 0.4])
+    nx, ny, nz, a = 8, 8, 6, 0.7
+    z = np.array([0.0, 0.1, 0.25, 0.45, 0.7, 1.0])
+    h = np.append(np.diff(z), z[-1] - z[-2])
+    o = Oracle(nx, ny, nz)
+    o.set_vertical_grid(z, h)
+    phi = np.ones((nz, ny, nx)) * (a * z ** 2)[:, None, None]
+    r, phi_bc = o.ma_residual(phi, np.full((nz, ny, nx), 2 * a), np.full((ny, nx), 2 * a), 0.0)
+    assert np.abs(r).max() < 1e-12
+    assert np.abs(phi_bc - phi).max() < 1e-13
+    # the centred :custom spacings are not the true spacings on this grid:
+    # the same profile then leaves a residual
+    _, dzc = make_vertical_grid(nz, 1.0, "custom", z)
+    o.set_vertical_grid(z, dzc)
+    r, _ = o.ma_residual(phi, np.full((nz, ny, nx), 2 * a), None, 0.0)
+    assert np.abs(r).max() > 1e-3
+
+
 def test_parsevalsum2_reference_weights():
     """src/transforms.jl:230-274: weights 1 at x-index 0 and Nx/2, 2 for 0 < i < Nx/2, 0 above."""
     nx, ny = 8, 8
