@@ -256,11 +256,16 @@ int ssg_set_b(ssg_ctx* ctx, const double* host, int64_t n, const ssg_ma_params* 
 /* compute_cfl_number (src/timestep.jl:91-117) on the surface velocities:
    cfl = max(max|u| dt/dx, max|v| dt/dy); umax, vmax optional. */
 int ssg_cfl(ssg_ctx* ctx, double dt, double* cfl, double* umax, double* vmax);
-/* Exact restart (no counterpart in the reference, whose checkpoint keeps b
-   and φ but not the AB2 history, src/output.jl:356-445): the step state
-   (b, φₛ, the AB2 history, t, step) as a byte blob.  Loading it into a
-   context of the same grid and precision resumes the frozen-φₛ step
-   bitwise. */
+/* Exact restart.  The reference's checkpoint (src/output.jl:318-448) keeps
+   t, step, dt, the grid, bₛ, φₛ and the 3D φ but not the AB2 history; this
+   byte blob holds the step state (b, φₛ, the AB2 history, t, step), the
+   last Monge-Ampère solve's Φ̂ (the 3D φ) and, for coupled stepping, the
+   coupling parameters and the warm-start state (the extrapolation ring of
+   converged D̂, up to nine nx/2+1 by ny by nz complex FP64 planes).
+   Loading it into a context of the same grid and precision restores the
+   coupling mode and resumes the frozen-φₛ or the coupled step bitwise; an
+   ssg_adjoint_coupled started after the load sees the same start state.
+   The size depends on what the context holds (ssg_state_size). */
 int ssg_state_size(ssg_ctx* ctx, size_t* bytes);
 int ssg_save_state(ssg_ctx* ctx, const ssg_time_state* state, void* buf, size_t bytes);
 int ssg_load_state(ssg_ctx* ctx, const void* buf, size_t bytes, ssg_time_state* state);

@@ -529,6 +529,7 @@ static int field_io(ssg_ctx* c, int32_t field, const double* src, double* dst, i
             c->ab2_have_history = false;
         } else if (field == SSG_FIELD_PHI) {
             c->ma_phi_current = false;
+            c->ma_phis_from_phi = false;
             HIP_OK(c, hipMemcpyAsync(c->phi, src, bytes, kind, c->stream));
             int rc = compute_velocities(c);
             if (rc) return rc;
@@ -671,9 +672,12 @@ int ssg_cfl(ssg_ctx* c, double dt, double* cfl, double* umax, double* vmax) {
     return SSG_OK;
 }
 
-// Step-state blob: header, b, φₛ (FP64) and the AB2 history in the path's
-// precision.  The velocities are recomputed from φₛ on load, as set_field
-// does, so a restart reproduces the uninterrupted frozen-φₛ run bitwise.
+// Step-state blob: header, b, φₛ (FP64), the AB2 history in the path's
+// precision, then the Monge-Ampère part (the last solve's Φ̂, the coupled
+// run's warm-start state and coupling parameters).  On load the velocities
+// are recomputed from φₛ, as set_field does, or from Φ̂ when φₛ came from
+// it, so a restart reproduces the uninterrupted run bitwise, frozen or
+// coupled.
 struct StateHeader {
     char magic[8];
     int32_t nx, ny, nz, precision;
@@ -683,11 +687,13 @@ struct StateHeader {
     double dt_actual;
 };
 
-static size_t state_bytes(const ssg_ctx* c) {
+static size_t state_bytes_base(const ssg_ctx* c) {
     const size_t n2 = (size_t)c->nx * c->ny;
     return sizeof(StateHeader) + 2 * n2 * sizeof(double) +
            n2 * (c->precision == SSG_FP32_MIXED ? sizeof(float) : sizeof(double));
 }
+
+static size_t state_bytes(const ssg_ctx* c) { return state_bytes_base(c) + ma_state_bytes(c); }
 
 int ssg_state_size(ssg_ctx* c, size_t* bytes) {
     if (!c || !bytes) return SSG_ERR_ARG;
@@ -702,7 +708,7 @@ int ssg_save_state(ssg_ctx* c, const ssg_time_state* s, void* buf, size_t bytes)
                                                state_bytes(c));
     HIP_OK(c, hipSetDevice(c->device));
     StateHeader h{};
-    memcpy(h.magic, "SSGSTAT1", 8);
+    memcpy(h.magic, "SSGSTAT2", 8);
     h.nx = c->nx;
     h.ny = c->ny;
     h.nz = c->nz;
@@ -719,10 +725,15 @@ int ssg_save_state(ssg_ctx* c, const ssg_time_state* s, void* buf, size_t bytes)
     p += n2 * sizeof(double);
     HIP_OK(c, hipMemcpyAsync(p, c->phi, n2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     p += n2 * sizeof(double);
-    if (c->precision == SSG_FP32_MIXED)
+    if (c->precision == SSG_FP32_MIXED) {
         HIP_OK(c, hipMemcpyAsync(p, c->f.fold, n2 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    else
+        p += n2 * sizeof(float);
+    } else {
         HIP_OK(c, hipMemcpyAsync(p, c->d.fold, n2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        p += n2 * sizeof(double);
+    }
+    int rc = ma_state_save(c, p);
+    if (rc) return rc;
     HIP_OK(c, hipStreamSynchronize(c->stream));
     return SSG_OK;
 }
@@ -733,12 +744,14 @@ int ssg_load_state(ssg_ctx* c, const void* buf, size_t bytes, ssg_time_state* s)
     StateHeader h{};
     if (bytes < sizeof(h)) return set_err(c, SSG_ERR_ARG, "state buffer too small");
     memcpy(&h, buf, sizeof(h));
-    if (memcmp(h.magic, "SSGSTAT1", 8) != 0) return set_err(c, SSG_ERR_ARG, "not a saved step state");
-    ma_history_reset(c);
+    const bool v1 = memcmp(h.magic, "SSGSTAT1", 8) == 0;
+    if (!v1 && memcmp(h.magic, "SSGSTAT2", 8) != 0) return set_err(c, SSG_ERR_ARG, "not a saved step state");
     if (h.nx != c->nx || h.ny != c->ny || h.precision != c->precision)
         return set_err(c, SSG_ERR_ARG, "state of a %dx%d grid (precision %d) does not fit this %dx%d context (%d)",
                        h.nx, h.ny, h.precision, c->nx, c->ny, c->precision);
-    if (bytes < state_bytes(c)) return set_err(c, SSG_ERR_ARG, "state buffer truncated");
+    if (bytes < state_bytes_base(c)) return set_err(c, SSG_ERR_ARG, "state buffer truncated");
+    HIP_OK(c, hipSetDevice(c->device));
+    ma_history_reset(c);
     const size_t n2 = (size_t)c->nx * c->ny;
     const char* p = static_cast<const char*>(buf) + sizeof(h);
     int rc = field_io(c, SSG_FIELD_B, reinterpret_cast<const double*>(p), nullptr, (int64_t)n2, false);
@@ -746,11 +759,18 @@ int ssg_load_state(ssg_ctx* c, const void* buf, size_t bytes, ssg_time_state* s)
     p += n2 * sizeof(double);
     if ((rc = field_io(c, SSG_FIELD_PHI, reinterpret_cast<const double*>(p), nullptr, (int64_t)n2, false))) return rc;
     p += n2 * sizeof(double);
-    if (c->precision == SSG_FP32_MIXED)
+    if (c->precision == SSG_FP32_MIXED) {
         HIP_OK(c, hipMemcpyAsync(c->f.fold, p, n2 * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    else
+        p += n2 * sizeof(float);
+    } else {
         HIP_OK(c, hipMemcpyAsync(c->d.fold, p, n2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        p += n2 * sizeof(double);
+    }
     HIP_OK(c, hipStreamSynchronize(c->stream));
+    if (!v1) {
+        const size_t used = (size_t)(p - static_cast<const char*>(buf));
+        if ((rc = ma_state_load(c, p, bytes - used))) return rc;
+    }
     c->ab2_have_history = h.have_ab2 != 0;
     s->t = h.t;
     s->step = h.step;

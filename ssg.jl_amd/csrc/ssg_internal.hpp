@@ -96,6 +96,8 @@ struct ssg_ctx {
     int ma_partial_used = 0;
     bool ma_have_d = false;
     bool ma_phi_current = false;      // coupled stepping: φₛ, u, v were solved for the current b with ma_par
+    bool ma_have_phi = false;         // ma_phi holds the last solve's Φ̂
+    bool ma_phis_from_phi = false;    // φₛ and the velocities are the top level of ma_phi (not set since)
     SolveHistory ma_hist;             // D̂ of the last converged solves (working buffer: ma_dprev)
     int ma_period = 0;                // solves per step of the running coupled stepper (RK3 3, AB2 1), else 0
     double ma_last_r0 = 1e300;        // first relative residual of the last solve
@@ -549,6 +551,9 @@ int rfft2_device(ssg_ctx* c, const double* src, cpx<double>* dst = nullptr);
 int ma_surface_outputs(ssg_ctx* c);
 void ma_free(ssg_ctx* c);
 void ma_history_reset(ssg_ctx* c);
+size_t ma_state_bytes(const ssg_ctx* c);
+int ma_state_save(ssg_ctx* c, char* p);
+int ma_state_load(ssg_ctx* c, const char* p, size_t avail);
 void adjc_free(ssg_ctx* c);
 int adj_alloc(ssg_ctx* c);
 

@@ -653,7 +653,139 @@ int ma_solve_dev(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_di
         if ((rc = ma_iterate(c, &cold_p, d))) return rc;
     }
     if ((rc = ma_surface_outputs(c))) return rc;
+    c->ma_have_phi = true;
+    c->ma_phis_from_phi = true;
     c->last_diag = *d;
+    return SSG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Monge-Ampère part of a saved step state: the last solve's Φ̂ (the 3D
+// streamfunction of the reference's checkpoint, src/output.jl:376-379) and
+// the warm-start state a coupled run continues from (the extrapolation
+// ring, or the last D̂), with the coupling parameters.  Restored, a coupled
+// run continues bitwise; the surface outputs are recomputed from Φ̂ by the
+// same passes that produced them.
+// ---------------------------------------------------------------------------
+struct MaStateHeader {
+    int32_t nz;
+    int32_t have_phi;     // Φ̂ follows (C3)
+    int32_t have_d;       // a warm start is held: ring entries, or the working D̂ when the ring is empty
+    int32_t ring_n;
+    int32_t ring_period;
+    int32_t phi_current;  // the coupled stepper's end-of-step solve is for the saved b
+    int32_t coupled;
+    int32_t phis_from_phi;  // φₛ and the velocities came from Φ̂'s top level (not set since)
+    double last_r0;
+    ssg_ma_params par;
+};
+
+static size_t ma_c3_bytes(const ssg_ctx* c) { return (size_t)c->ncols_full * c->nz * c->ny * sizeof(cpx<double>); }
+
+static int ma_state_nbuf(const ssg_ctx* c) {
+    if (!c->ma_ready) return 0;
+    int n = c->ma_have_phi ? 1 : 0;
+    if (c->ma_have_d) n += c->ma_hist.n > 0 ? c->ma_hist.n : 1;
+    return n;
+}
+
+size_t ma_state_bytes(const ssg_ctx* c) { return sizeof(MaStateHeader) + ma_state_nbuf(c) * ma_c3_bytes(c); }
+
+// header and device buffers into p (device-to-host copies on the context
+// stream; the caller synchronises)
+int ma_state_save(ssg_ctx* c, char* p) {
+    MaStateHeader h{};
+    h.nz = c->nz;
+    h.have_phi = c->ma_ready && c->ma_have_phi;
+    h.have_d = c->ma_ready && c->ma_have_d;
+    h.ring_n = h.have_d ? c->ma_hist.n : 0;
+    h.ring_period = c->ma_hist.period;
+    h.phi_current = c->ma_phi_current;
+    h.coupled = c->coupled;
+    h.phis_from_phi = c->ma_phis_from_phi;
+    h.last_r0 = c->ma_last_r0;
+    h.par = c->ma_par;
+    memcpy(p, &h, sizeof(h));
+    p += sizeof(h);
+    const size_t c3b = ma_c3_bytes(c);
+    auto put = [&](const cpx<double>* src) -> int {
+        HIP_OK(c, hipMemcpyAsync(p, src, c3b, hipMemcpyDeviceToHost, c->stream));
+        p += c3b;
+        return SSG_OK;
+    };
+    int rc;
+    if (h.have_phi && (rc = put(c->ma_phi))) return rc;
+    if (h.have_d) {
+        if (h.ring_n > 0) {
+            for (int i = 0; i < h.ring_n; ++i)
+                if ((rc = put(c->ma_hist.ring[i]))) return rc;
+        } else if ((rc = put(c->ma_dprev))) {
+            return rc;
+        }
+    }
+    return SSG_OK;
+}
+
+int ma_state_load(ssg_ctx* c, const char* p, size_t avail) {
+    MaStateHeader h{};
+    if (avail < sizeof(h)) return set_err(c, SSG_ERR_ARG, "state buffer truncated (Monge-Ampere part)");
+    memcpy(&h, p, sizeof(h));
+    p += sizeof(h);
+    avail -= sizeof(h);
+    ma_history_reset(c);
+    c->ma_have_d = false;
+    c->ma_have_phi = false;
+    c->coupled = false;
+    const int nbuf = (h.have_phi ? 1 : 0) + (h.have_d ? (h.ring_n > 0 ? h.ring_n : 1) : 0);
+    if (nbuf == 0 && !h.coupled) return SSG_OK;
+    if (h.nz != c->nz)
+        return set_err(c, SSG_ERR_ARG, "state of nz = %d does not fit this context (nz = %d)", h.nz, c->nz);
+    if (h.ring_n < 0 || h.ring_n > 9) return set_err(c, SSG_ERR_ARG, "corrupt state (ring of %d)", h.ring_n);
+    int rc = ma_alloc(c);
+    if (rc) return rc;
+    const size_t c3b = ma_c3_bytes(c);
+    if (avail < nbuf * c3b) return set_err(c, SSG_ERR_ARG, "state buffer truncated (Monge-Ampere planes)");
+    auto get = [&](cpx<double>* dst) -> int {
+        HIP_OK(c, hipMemcpyAsync(dst, p, c3b, hipMemcpyHostToDevice, c->stream));
+        p += c3b;
+        return SSG_OK;
+    };
+    if (h.have_phi && (rc = get(c->ma_phi))) return rc;
+    SolveHistory& hs = c->ma_hist;
+    if (h.have_d) {
+        if (h.ring_n > 0) {
+            for (int i = 0; i < h.ring_n; ++i) {
+                cpx<double>* buf = nullptr;
+                if (hs.nspare > 0)
+                    buf = hs.spare[--hs.nspare];
+                else
+                    HIP_OK(c, hipMalloc(&buf, c3b));
+                hs.ring[i] = buf;
+                hs.n = i + 1;
+                if ((rc = get(buf))) return rc;
+            }
+            hs.period = h.ring_period;
+        } else if ((rc = get(c->ma_dprev))) {
+            return rc;
+        }
+        c->ma_have_d = true;
+    }
+    c->ma_last_r0 = h.last_r0;
+    if (h.coupled) {
+        if ((rc = ma_check_params(c, &h.par))) return rc;
+        c->ma_par = h.par;
+        c->coupled = true;
+    }
+    if (h.have_phi) {
+        c->ma_have_phi = true;
+        if (h.phis_from_phi) {
+            // φₛ and the velocities by the passes that made them after the solve
+            if ((rc = ma_surface_outputs(c))) return rc;
+            c->ma_phis_from_phi = true;
+            c->ma_phi_current = h.phi_current != 0;
+        }
+    }
+    HIP_OK(c, hipStreamSynchronize(c->stream));
     return SSG_OK;
 }
 
@@ -1374,6 +1506,11 @@ int ssg_set_vertical_grid(ssg_ctx* c, const double* z, const double* dz, int32_t
 
 int ssg_set_coupling(ssg_ctx* c, const ssg_ma_params* p) {
     if (!c) return SSG_ERR_ARG;
+    // the same parameters again (e.g. after a restart that restored them)
+    // leave the coupled run's history in place
+    if (p && c->coupled && p->eps == c->ma_par.eps && p->tol == c->ma_par.tol && p->maxiter == c->ma_par.maxiter &&
+        p->variant == c->ma_par.variant && p->warm == c->ma_par.warm)
+        return SSG_OK;
     ma_history_reset(c);
     if (!p) {
         c->coupled = false;

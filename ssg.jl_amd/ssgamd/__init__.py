@@ -550,6 +550,8 @@ def set_vertical_grid(domain: Domain, z, dz=None) -> None:
         dz = np.ascontiguousarray(dz, dtype=np.float64)
         dzp = dz.ctypes.data_as(_dp)
     domain.check(domain.lib.ssg_set_vertical_grid(domain.handle, z.ctypes.data_as(_dp), dzp, int(z.size)))
+    domain._z = z.copy()
+    domain._dz = dz.copy() if dz is not None else make_vertical_grid(z.size, float(z[-1] - z[0]), "custom", z)[1]
 
 
 def get_phi3d(domain: Domain) -> np.ndarray:
@@ -604,9 +606,11 @@ def compute_cfl_number(fields: Fields, domain: Domain, dt: float) -> float:
 
 
 def save_state(domain: Domain, state: "TimeState") -> bytes:
-    """The step state (b, φₛ, the AB2 history, t and step) as bytes; loading
-    it into a context of the same grid and precision resumes bitwise for the
-    frozen-φₛ step."""
+    """The exact-restart state (``ssg_save_state``) as bytes: b, φₛ, the AB2
+    history, t and step, the last Monge-Ampère solve's Φ̂ and, in coupled
+    mode, the coupling parameters and warm-start history.  Loading it into a
+    context of the same grid and precision resumes bitwise, frozen or
+    coupled."""
     n = ctypes.c_size_t()
     domain.check(domain.lib.ssg_state_size(domain.handle, ctypes.byref(n)))
     buf = ctypes.create_string_buffer(n.value)
@@ -614,9 +618,65 @@ def save_state(domain: Domain, state: "TimeState") -> bytes:
     return buf.raw
 
 
+_STATE_HEADER_BYTES = 56
+
+
 def load_state(domain: Domain, blob: bytes, state: "TimeState") -> None:
     buf = ctypes.create_string_buffer(blob, len(blob))
     domain.check(domain.lib.ssg_load_state(domain.handle, buf, len(blob), ctypes.byref(state._c)))
+    # the blob restores the coupling mode; mirror it
+    n2 = domain.Nx * domain.Ny
+    ma = _STATE_HEADER_BYTES + n2 * (16 + (4 if domain.precision == SSG_FP32_MIXED else 8))
+    if blob[:8] == b"SSGSTAT2" and len(blob) >= ma + 28:
+        domain.coupled = bool(int.from_bytes(blob[ma + 24:ma + 28], "little"))
+
+
+def save_checkpoint(path: str, fields: Fields, domain: Domain, state: "TimeState",
+                    params: "TimeParams | None" = None) -> str:
+    """``save_simulation_state_full`` (src/output.jl:318-448) with the
+    reference's keys (time, step, dt, grid/*, fields/surface/*, fields/3d/*,
+    timestepper/*), written as a numpy ``.npz`` archive (no JLD2/HDF5 writer
+    is available here), plus ``restart/state``, the exact-restart bytes of
+    :func:`save_state`.  Arrays keep this library's (Ny, Nx[, Nz-first])
+    C order, which is the reference's column-major (Nx, Ny[, Nz]) memory."""
+    d = domain
+    z, dz = (d._z, d._dz) if getattr(d, "_z", None) is not None else make_vertical_grid(d.Nz, d.Lz)
+    out = {
+        "time": np.float64(state.t), "step": np.int64(state.step),
+        "dt": np.float64(state.dt_actual if state.dt_actual > 0 else (params.dt if params else 0.0)),
+        "grid/Nx": np.int64(d.Nx), "grid/Ny": np.int64(d.Ny), "grid/Nz": np.int64(d.Nz),
+        "grid/Lx": np.float64(d.Lx), "grid/Ly": np.float64(d.Ly), "grid/Lz": np.float64(d.Lz),
+        "grid/x": d.Lx / d.Nx * np.arange(d.Nx), "grid/y": d.Ly / d.Ny * np.arange(d.Ny),
+        "grid/z": np.asarray(z), "grid/dz": np.asarray(dz),
+        "fields/surface/buoyancy": fields.bs, "fields/surface/streamfunction": fields.phis,
+        "restart/state": np.frombuffer(save_state(d, state), dtype=np.uint8),
+    }
+    u, v = fields.u, fields.v
+    out["fields/3d/u_velocity"] = np.broadcast_to(u, (d.Nz,) + u.shape).copy()   # every level (src/setICs.jl:161-192)
+    out["fields/3d/v_velocity"] = np.broadcast_to(v, (d.Nz,) + v.shape).copy()
+    try:
+        out["fields/3d/streamfunction"] = get_phi3d(d)
+    except SSGError:
+        pass   # no Monge-Ampère solve yet
+    if params is not None:
+        out.update({"timestepper/scheme": np.array(["AB2_LowStorage", "RK3", "RK3_LowStorage"][params.scheme]),
+                    "timestepper/dt": np.float64(params.dt), "timestepper/adaptive_dt": np.bool_(params.adaptive_dt),
+                    "timestepper/filter_freq": np.int64(params.filter_freq),
+                    "timestepper/filter_strength": np.float64(params.filter_strength),
+                    "timestepper/cfl_safety": np.float64(params.cfl_safety),
+                    "timestepper/max_dt": np.float64(params.max_dt), "timestepper/min_dt": np.float64(params.min_dt)})
+    with open(path, "wb") as fh:
+        np.savez(fh, **out)
+    return path
+
+
+def load_checkpoint(path: str, domain: Domain, state: "TimeState") -> dict:
+    """Restores the exact-restart state saved by :func:`save_checkpoint` into
+    ``domain`` and ``state``; returns the archive's other entries."""
+    with np.load(path, allow_pickle=False) as z:
+        data = {k: z[k] for k in z.files}
+    load_state(domain, data.pop("restart/state").tobytes(), state)
+    return data
 
 
 def step_until(fields: Fields, domain: Domain, params: "TimeParams", state: "TimeState", stop_time: float) -> int:

@@ -113,6 +113,79 @@ def test_restart_is_bitwise(gpu, prec, scheme):
     d2.close()
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_coupled_restart_is_bitwise(gpu, prec, scheme, tmp_path):
+    """Coupled stepping (a Monge-Ampère solve per stage, extrapolated warm
+    starts): checkpointed after 7 steps, when the same-stage extrapolation
+    ring is full, and loaded into a fresh context, 6 more steps give bitwise
+    the uninterrupted run's b, φₛ and 3D φ.  The checkpoint carries the
+    reference's keys (src/output.jl:356-409); re-stating the same coupling
+    after the load keeps the restored history."""
+    from ssgamd.ics import spectral_b0
+    n, nz, eps = 128, 8, 0.1
+    b0 = spectral_b0(n, n, seed=5)
+    p = gpu.TimeParams(1e-3, scheme=scheme, filter_freq=3, filter_strength=36.0, filter_order=8, filter_cutoff=0.65)
+    d = gpu.make_domain(n, n, nz, precision=prec)
+    f = gpu.allocate_fields(d)
+    f.bs = b0
+    gpu.set_coupling(d, eps=eps, tol=1e-10, maxiter=60, variant=gpu.MA_BODY, warm=2)
+    st = gpu.TimeState(0.0, f)
+    gpu.timestep(f, d, p, st, 7)
+    path = gpu.save_checkpoint(str(tmp_path / "ck.npz"), f, d, st, p)
+    gpu.timestep(f, d, p, st, 6)
+    ref_b, ref_phis, ref_phi = f.bs, f.phis, gpu.get_phi3d(d)
+    d.close()
+    for restate in (False, True):
+        d2 = gpu.make_domain(n, n, nz, precision=prec)
+        f2 = gpu.allocate_fields(d2)
+        st2 = gpu.TimeState(0.0, f2)
+        data = gpu.load_checkpoint(path, d2, st2)
+        assert d2.coupled and st2.step == 7
+        for k in ("time", "step", "dt", "grid/Nx", "grid/z", "grid/dz", "fields/surface/buoyancy",
+                  "fields/surface/streamfunction", "fields/3d/streamfunction", "fields/3d/u_velocity",
+                  "timestepper/scheme"):
+            assert k in data, k
+        assert data["fields/3d/streamfunction"].shape == (nz, n, n)
+        if restate:
+            gpu.set_coupling(d2, eps=eps, tol=1e-10, maxiter=60, variant=gpu.MA_BODY, warm=2)
+        gpu.timestep(f2, d2, p, st2, 6)
+        assert np.array_equal(f2.bs, ref_b), restate
+        assert np.array_equal(f2.phis, ref_phis), restate
+        assert np.array_equal(gpu.get_phi3d(d2), ref_phi), restate
+        d2.close()
+
+
+def test_coupled_adjoint_from_restart(gpu):
+    """The coupled adjoint started after loading a checkpoint equals the one
+    started in the run that wrote it (same start state, bitwise)."""
+    from ssgamd.ics import spectral_b0
+    n, nz, eps, dt = 64, 6, 0.1, 2e-3
+    b0 = spectral_b0(n, n, seed=2, k0=4.0)
+    lam = spectral_b0(n, n, seed=3, k0=4.0)
+    p = gpu.TimeParams(dt, scheme=gpu.RK3, filter_freq=2, filter_strength=36.0, filter_order=8, filter_cutoff=0.65)
+    d = gpu.make_domain(n, n, nz)
+    f = gpu.allocate_fields(d)
+    f.bs = b0
+    gpu.set_coupling(d, eps=eps, tol=1e-12, maxiter=100, variant=gpu.MA_BODY, warm=2)
+    st = gpu.TimeState(0.0, f)
+    gpu.timestep(f, d, p, st, 3)
+    blob = gpu.save_state(d, st)
+    gpu.set_coupling(d, eps=None)
+    g1 = gpu.timestep_adjoint_coupled(d, p, 3, 4, lam, eps=eps, tol=1e-12, maxiter=100, variant=gpu.MA_BODY,
+                                      warm=2, checkpoint_every=2)
+    d.close()
+    d2 = gpu.make_domain(n, n, nz)
+    f2 = gpu.allocate_fields(d2)
+    st2 = gpu.TimeState(0.0, f2)
+    gpu.load_state(d2, blob, st2)
+    gpu.set_coupling(d2, eps=None)
+    g2 = gpu.timestep_adjoint_coupled(d2, p, st2.step, 4, lam, eps=eps, tol=1e-12, maxiter=100,
+                                      variant=gpu.MA_BODY, warm=2, checkpoint_every=2)
+    d2.close()
+    assert np.array_equal(g1, g2)
+
+
 def test_diagnostic_time_series(gpu):
     from ssgamd.ics import spectral_b0, streamfunction_from_b
     n = 64
