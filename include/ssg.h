@@ -213,6 +213,17 @@ int ssg_ma_solve(ssg_ctx* ctx, const ssg_ma_params* params, ssg_ma_diag* diag);
    (src/poisson.jl:309-364) do.  Without this call the grid is uniform,
    dz = Lz/(Nz-1) (src/domain.jl:238-249). */
 int ssg_set_vertical_grid(ssg_ctx* ctx, const double* z, const double* dz, int32_t nz);
+/* Vertical velocity from the ω equation (compute_vertical_velocity,
+   src/extras/omega_eq.jl:574-652) for the current bₛ: Φ from the
+   Monge-Ampère solve with params (the ω equation's Φ is the surface-forced
+   problem, SSG_MA_SURFACE), Q = (∂x u_g ∂x b + ∂x v_g ∂y b,
+   ∂y u_g ∂x b + ∂y v_g ∂y b) per level (compute_divergence_Q!, :326-405),
+   (∇ₕ² + ∂zz) w* = -2ε ∇·Q per horizontal wavenumber on the solve's vertical
+   stencil with w* = 0 on the bottom and top levels, and w = w* / (1 - ε∇ₕ²Φ)
+   (:612-646).  w: nx*ny*nz, x fastest then y then z.  The reference's own
+   vertical integrals (:408-521) do not satisfy that equation; this is the
+   equation its documentation states. */
+int ssg_vertical_velocity(ssg_ctx* ctx, const ssg_ma_params* params, double* w, int64_t n, ssg_ma_diag* diag);
 /* fields.φ (nx*ny*nz, x fastest then y then z) after ssg_ma_solve */
 int ssg_get_phi3d(ssg_ctx* ctx, double* host, int64_t n);
 /* Coupled stepping: with params != NULL, ssg_timestep re-solves the

@@ -108,6 +108,12 @@ struct ssg_ctx {
     int ma_ept[2] = {0, 0};           // elements per thread of k_ma_rows per path (0 FP64, 1 FP32)
     Grid g3{};                        // 3D plane geometry: pitch = nz * ny
     std::vector<double> dz;           // vertical spacings (Nz entries) of a custom grid; empty: uniform Lz/(Nz-1)
+    // vertical velocity (allocated on first use): Φxx, Φyy, Φxy per level,
+    // (Q₁, Q₂) per level, the surface pair (-∂y b, ∂x b), w
+    double* om_pd[3] = {nullptr, nullptr, nullptr};
+    double2* om_q = nullptr;
+    double2* om_gb = nullptr;
+    double* om_w = nullptr;
     // coupled stepping (MA solve at every stage)
     bool coupled = false;
     ssg_ma_params ma_par{};
@@ -272,20 +278,24 @@ int ept_of(ssg_ctx* c, int kind, int n) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// rows > 0 with `grid` transforms that many rows of a stacked field (the
+// levels of a 3D field, grid pitch nz * ny)
 template <typename R, typename S, int SRC>
-int launch_rows_fwd(ssg_ctx* c, const S* f, const cpx<R>* w, cpx<R>* PA, cpx<R>* PB, int ncols) {
+int launch_rows_fwd(ssg_ctx* c, const S* f, const cpx<R>* w, cpx<R>* PA, cpx<R>* PB, int ncols, int rows = 0,
+                    const Grid* grid = nullptr) {
     PathBuffers<R>& B = bufs<R>(c);
     const int ept = ept_of<R>(c, 0, c->nx);
+    if (rows <= 0) rows = c->ny;
     return with_size_ept(c->nx, ept, [&](auto tag, auto etag) -> int {
         constexpr int N = decltype(tag)::value;
         constexpr int E = decltype(etag)::value;
         constexpr int T = N / E;
         const int rows_per_line = SRC == SRC_PAIR ? 2 : 1;
-        const int G = lines_per_block<R>(N, E, c->ny / rows_per_line);
-        const int nblk = c->ny / (rows_per_line * G);
-        RowsFwdArgs<R, S> a{f, w, PA, PB, ncols, G, nblk, B.twx, c->g};
+        const int G = lines_per_block<R>(N, E, rows / rows_per_line);
+        const int nblk = rows / (rows_per_line * G);
+        RowsFwdArgs<R, S> a{f, w, PA, PB, ncols, G, nblk, B.twx, grid ? *grid : c->g};
         const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<R>);
-        const double n2 = (double)c->nx * c->ny, plane = (double)ncols * c->ny * sizeof(cpx<R>);
+        const double n2 = (double)c->nx * rows, plane = (double)ncols * rows * sizeof(cpx<R>);
         const double bytes = SRC == SRC_FLUX ? n2 * (sizeof(double) + sizeof(cpx<R>)) + 2 * plane
                              : SRC == SRC_W  ? n2 * sizeof(cpx<R>) + 2 * plane
                                              : n2 * sizeof(S) + plane;

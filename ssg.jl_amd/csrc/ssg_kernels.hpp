@@ -62,7 +62,8 @@ enum {
     OP_ADJ = 13,       // A = Dx M P, B = Dy M P (transposed tendency, spectral part)
     OP_PHIBAR = 14,    // y-FFT of (U, V) -> Dy U - Dx V, spectral (adjoint of u = -Dy φ, v = Dx φ)
     OP_MA_ADJ = 15,    // y-FFT of three products -> Dxx P1 + Dyy P2 - 2 Dxy P3 on interior levels, residual sums
-    OP_SETB = 16       // M P with the k = 0 mode removed (set_b!'s zero mean and dealias)
+    OP_SETB = 16,      // M P with the k = 0 mode removed (set_b!'s zero mean and dealias)
+    OP_DIVQ = 17       // y-FFT of (Q₁, Q₂) -> -2ε (Dx Q₁ + Dy Q₂), left spectral (ω equation)
 };
 
 struct Grid {
@@ -336,7 +337,7 @@ __host__ __device__ constexpr int cols_nbuf(int op) {
     return (op == OP_MA_DERIV || op == OP_MA_ADJ)
                ? 3
                : ((op == OP_TEND || op == OP_TEND_Q || op == OP_JAC || op == OP_VEL || op == OP_VEL_SPEC ||
-                   op == OP_ADJ || op == OP_PHIBAR)
+                   op == OP_ADJ || op == OP_PHIBAR || op == OP_DIVQ)
                       ? 2
                       : 1);
 }
@@ -358,7 +359,8 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
     C* const buf1[1] = {lds + (l * NBUF) * LE};
     typename F::Twiddles tw;
     tw.load(t, a.twy);
-    constexpr bool TWO_IN = (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC || OP == OP_PHIBAR || OP == OP_MA_ADJ);
+    constexpr bool TWO_IN =
+        (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC || OP == OP_PHIBAR || OP == OP_MA_ADJ || OP == OP_DIVQ);
     C vv[2][E];
     C (&va)[E] = vv[0];
     C (&vb)[E] = vv[1];
@@ -432,6 +434,23 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
             if (live) dp[y] = q;
         }
         block_sum_g<2>(acc, a.partial, bidx);
+        return;
+    } else if constexpr (OP == OP_DIVQ) {
+        // -2ε ∇·Q (src/extras/omega_eq.jl:400-402), left spectral: -2ε (i kx Q̂₁ + i ky Q̂₂),
+        // without the x- and y-Nyquist bins (where the transform conventions of
+        // that file and of the step path disagree)
+        C* const b2[2] = {lds + (l * NBUF) * LE, lds + (l * NBUF + 1) * LE};
+        F::template run_n<-1, 2>(vv, b2, t, tw);
+        const R f = (R)(-2.0 * a.eps);
+        const bool xnyq = 2 * c == a.g.nx;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int s = signed_k(t + T * m, a.g.ny);
+            const C px = cmul_i(va[m], (R)(c * a.g.dkx)), py = cmul_i(vb[m], (R)(s * a.g.dky));
+            const R g = (xnyq || 2 * s == -a.g.ny) ? (R)0 : f;
+            va[m] = C{g * (px.x + py.x), g * (px.y + py.y)};
+        }
+        store(a.outA, va);
         return;
     } else if constexpr (OP == OP_PHIBAR) {
         // φ̄ = Dy U - Dx V (u = -Dy φ, v = Dx φ; Dx, Dy are antisymmetric),

@@ -488,3 +488,93 @@ static __global__ __launch_bounds__(kMaSumThreads) void k_ma_sum(const double2* 
 }
 
 }  // namespace ssg
+
+namespace ssg {
+
+// ---------------------------------------------------------------------------
+// Vertical velocity (ω equation, src/extras/omega_eq.jl:326-652) on the
+// Monge-Ampère planes.
+//   Q = (Q₁, Q₂),  Q₁ = ∂x u_g ∂x b + ∂x v_g ∂y b,  Q₂ = ∂y u_g ∂x b + ∂y v_g ∂y b
+//   with u_g = -∂yΦ, v_g = ∂xΦ per level and b the surface buoyancy
+//   (compute_divergence_Q!, :326-405);
+//   (∂xx + ∂yy + ∂zz) w* = -2ε ∇·Q with w* = 0 on the bottom and top levels,
+//   solved per horizontal wavenumber on the Monge-Ampère vertical stencil;
+//   w = w* / (1 - ε (Φxx + Φyy))  (the inverse Jacobian, :612-635).
+// ---------------------------------------------------------------------------
+
+// (Q₁, Q₂) at every point of every level from the physical derivative
+// fields [lev][y][x] and the surface pair gb = (-∂y b, ∂x b) [y][x]
+static __global__ __launch_bounds__(256) void k_omega_q(const double* __restrict__ pxx, const double* __restrict__ pyy,
+                                                        const double* __restrict__ pxy,
+                                                        const double2* __restrict__ gb, double2* __restrict__ q,
+                                                        long n2, long n3) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= n3) return;
+    const double2 g = gb[i % n2];
+    const double bx = g.y, by = -g.x;
+    const double xx = pxx[i], yy = pyy[i], xy = pxy[i];
+    // ∂x u_g = -Φxy, ∂x v_g = Φxx, ∂y u_g = -Φyy, ∂y v_g = Φxy
+    q[i] = double2{-xy * bx + xx * by, -yy * bx + xy * by};
+}
+
+struct OmegaTriArgs {
+    const cpx<double>* F;  // -2ε ∇·Q, [kx][lev][ky] (forward spectrum)
+    cpx<double>* W;        // w*, same layout
+    const double* coef;    // the vertical stencil (k_ma_tridiag's table)
+    int nz;
+    int ncols;
+    Grid g;
+};
+
+// Per (kx, ky): α_k w_{k-1} + (β_k - |k|²) w_k + γ_k w_{k+1} = f_k on the
+// interior levels, w = 0 on the bottom and top levels (Thomas sweep).
+template <int NZMAX>
+__global__ __launch_bounds__(128) void k_omega_tridiag(OmegaTriArgs a) {
+    const long total = (long)a.ncols * a.g.ny;
+    const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    const int ny = a.g.ny, nz = a.nz;
+    const int kx = (int)(q / ny), ky = (int)(q - (long)kx * ny);
+    const int s = signed_k(ky, ny);
+    const double hs = (double)sym_dxx<double>(a.g, kx) + (double)sym_dyy<double>(a.g, s);
+    const double* cf = a.coef;
+    const size_t base = (size_t)kx * nz * ny + ky;
+    double cp[NZMAX];
+    double2 dp[NZMAX];
+#pragma unroll
+    for (int k = 1; k < NZMAX; ++k)
+        if (k < nz - 1) dp[k] = a.F[base + (size_t)k * ny];
+    cp[0] = 0.0;
+    dp[0] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 1; k < NZMAX; ++k) {
+        if (k < nz - 1) {
+            const double al = cf[3 * k], be = cf[3 * k + 1] + hs, ga = cf[3 * k + 2];
+            const double2 f = dp[k];
+            // row 1 sees w_0 = 0 (cp[0] = dp[0] = 0); the last interior row's γ term meets w_{nz-1} = 0
+            const double rm = 1.0 / (be - al * cp[k - 1]);
+            cp[k] = (k < nz - 2) ? ga * rm : 0.0;
+            dp[k] = {(f.x - al * dp[k - 1].x) * rm, (f.y - al * dp[k - 1].y) * rm};
+        }
+    }
+    double2 x = {0.0, 0.0};
+    a.W[base + (size_t)(nz - 1) * ny] = cpx<double>{0.0, 0.0};
+#pragma unroll
+    for (int k = NZMAX - 1; k >= 1; --k) {
+        if (k < nz - 1) {
+            x = {dp[k].x - cp[k] * x.x, dp[k].y - cp[k] * x.y};
+            a.W[base + (size_t)k * ny] = x;
+        }
+    }
+    a.W[base] = cpx<double>{0.0, 0.0};
+}
+
+// w = w* / (1 - ε (Φxx + Φyy)), in place
+static __global__ __launch_bounds__(256) void k_omega_final(double* __restrict__ w, const double* __restrict__ pxx,
+                                                            const double* __restrict__ pyy, double eps, long n3) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= n3) return;
+    w[i] = w[i] / (1.0 - eps * (pxx[i] + pyy[i]));
+}
+
+}  // namespace ssg

@@ -92,7 +92,15 @@ int ma_alloc(ssg_ctx* c) {
     return SSG_OK;
 }
 
+void om_free(ssg_ctx* c) {
+    for (int i = 0; i < 3; ++i) (void)hipFree(c->om_pd[i]);
+    (void)hipFree(c->om_q);
+    (void)hipFree(c->om_gb);
+    (void)hipFree(c->om_w);
+}
+
 void ma_free(ssg_ctx* c) {
+    om_free(c);
     (void)hipFree(c->ma_phi);
     (void)hipFree(c->ma_dprev);
     for (int i = 0; i < 3; ++i) (void)hipFree(c->ma_p[i]);
@@ -1383,9 +1391,138 @@ int adjoint_coupled(ssg_ctx* c, const ssg_time_params* p, const ssg_ma_params* m
     return rc;
 }
 
+// ---------------------------------------------------------------------------
+// Vertical velocity from the ω equation (src/extras/omega_eq.jl:574-652,
+// compute_vertical_velocity) for the current surface buoyancy: Φ from the
+// Monge-Ampère solve, Q and -2ε∇·Q per level, the per-wavenumber vertical
+// solve with w* = 0 on the bottom and top levels, w = w*/(1 - ε∇ₕ²Φ).
+// ---------------------------------------------------------------------------
+int omega_alloc(ssg_ctx* c) {
+    if (c->om_w) return SSG_OK;
+    const size_t n2 = (size_t)c->nx * c->ny, n3 = n2 * c->nz;
+    for (int i = 0; i < 3; ++i) HIP_OK(c, hipMalloc(&c->om_pd[i], n3 * sizeof(double)));
+    HIP_OK(c, hipMalloc(&c->om_q, n3 * sizeof(double2)));
+    HIP_OK(c, hipMalloc(&c->om_gb, n2 * sizeof(double2)));
+    HIP_OK(c, hipMalloc(&c->om_w, n3 * sizeof(double)));
+    return SSG_OK;
+}
+
+int omega_w(ssg_ctx* c, double eps) {
+    PathBuffers<double>& D = c->d;
+    const double scale = 1.0 / ((double)c->nx * c->ny);
+    const int lines = c->ncols_full * c->nz, rows = c->nz * c->ny;
+    const long n2 = (long)c->nx * c->ny, n3 = n2 * c->nz;
+    // Φxx, Φyy, Φxy of every level in physical space
+    ColsArgs<double> a{};
+    a.inA = c->ma_phi;
+    a.outA = c->ma_p[0];
+    a.outB = c->ma_p[1];
+    a.outC = c->ma_p[2];
+    a.scale = scale;
+    int rc = launch_cols_args<double, OP_MA_DERIV>(c, a, lines, c->nz);
+    if (rc) return rc;
+    rc = with_size_ept(c->nx, ma_ept_of<double>(c), [&](auto tag, auto etag) -> int {
+        constexpr int N = decltype(tag)::value;
+        constexpr int E = decltype(etag)::value;
+        constexpr int T = N / E;
+        const int G = lines_per_block<double>(N, E, rows / 2, 1);
+        const int nblk = rows / (2 * G);
+        MaRowsArgs<double> m{c->ma_p[0], c->ma_p[1], c->ma_p[2], nullptr, {c->om_pd[0], c->om_pd[1], c->om_pd[2]},
+                             c->ncols_full, G, nblk, D.twx, c->g3};
+        const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<double>);
+        hipLaunchKernelGGL((k_ma_rows<double, N, E, MA_ROWS_PHYS>), dim3(nblk), dim3(G * T), lds, c->stream, m);
+        c->launches++;
+        HIP_OK(c, hipGetLastError());
+        return SSG_OK;
+    });
+    if (rc) return rc;
+    // (-∂y b, ∂x b) of the surface buoyancy, as the velocities are made from φₛ
+    if ((rc = launch_rows_fwd<double, double, SRC_PAIR>(c, c->b, nullptr, D.Fx, nullptr, c->ncols_full))) return rc;
+    if ((rc = launch_cols<double, OP_VEL>(c, D.Fx, nullptr, D.Tp, D.Fy, c->ncols_full, scale))) return rc;
+    RowsInvArgs<double> g{};
+    g.P0 = D.Tp;
+    g.P1 = D.Fy;
+    g.ncols_in = c->ncols_full;
+    g.w_out = c->om_gb;
+    if ((rc = launch_rows_inv<double, MODE_PLANES, EPI_STORE_W>(c, g))) return rc;
+    // Q per level, x-FFT into two planes, y-FFT and -2ε ∇·Q
+    hipLaunchKernelGGL(k_omega_q, dim3(grid_all(n3)), dim3(256), 0, c->stream, c->om_pd[0], c->om_pd[1], c->om_pd[2],
+                       (const double2*)c->om_gb, c->om_q, n2, n3);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    if ((rc = launch_rows_fwd<double, double, SRC_W>(c, nullptr, c->om_q, c->ma_p[0], c->ma_p[1], c->ncols_full, rows,
+                                                      &c->g3)))
+        return rc;
+    ColsArgs<double> dq{};
+    dq.inA = c->ma_p[0];
+    dq.inB = c->ma_p[1];
+    dq.outA = c->ma_p[2];
+    dq.eps = eps;
+    if ((rc = launch_cols_args<double, OP_DIVQ>(c, dq, lines, c->nz))) return rc;
+    // per-wavenumber vertical solve
+    OmegaTriArgs t{};
+    t.F = c->ma_p[2];
+    t.W = c->ma_p[1];
+    t.coef = c->ma_coef;
+    t.nz = c->nz;
+    t.ncols = c->ncols_full;
+    t.g = c->g;
+    const long total = (long)c->ncols_full * c->ny;
+    const int nblk = (int)((total + 127) / 128);
+    if (c->nz <= 8)
+        hipLaunchKernelGGL(k_omega_tridiag<8>, dim3(nblk), dim3(128), 0, c->stream, t);
+    else if (c->nz <= 16)
+        hipLaunchKernelGGL(k_omega_tridiag<16>, dim3(nblk), dim3(128), 0, c->stream, t);
+    else if (c->nz <= 32)
+        hipLaunchKernelGGL(k_omega_tridiag<32>, dim3(nblk), dim3(128), 0, c->stream, t);
+    else if (c->nz <= 64)
+        hipLaunchKernelGGL(k_omega_tridiag<64>, dim3(nblk), dim3(128), 0, c->stream, t);
+    else
+        hipLaunchKernelGGL(k_omega_tridiag<128>, dim3(nblk), dim3(128), 0, c->stream, t);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    // w* per level in physical space, then w = w*/J⁻¹
+    ColsArgs<double> iv{};
+    iv.inA = c->ma_p[1];
+    iv.outA = c->ma_p[0];
+    iv.scale = scale;
+    if ((rc = launch_cols_args<double, OP_INV>(c, iv, lines, c->nz))) return rc;
+    RowsInvArgs<double> r{};
+    r.P0 = c->ma_p[0];
+    r.ncols_in = c->ncols_full;
+    r.out_r = c->om_w;
+    if ((rc = launch_rows_inv<double, MODE_PAIR, EPI_STORE>(c, r, rows, &c->g3))) return rc;
+    hipLaunchKernelGGL(k_omega_final, dim3(grid_all(n3)), dim3(256), 0, c->stream, c->om_w, c->om_pd[0], c->om_pd[1],
+                       eps, n3);
+    c->launches++;
+    HIP_OK(c, hipGetLastError());
+    return SSG_OK;
+}
+
 }  // namespace ssg_impl
 
 extern "C" {
+
+int ssg_vertical_velocity(ssg_ctx* c, const ssg_ma_params* p, double* w, int64_t n, ssg_ma_diag* diag) {
+    if (!c) return SSG_ERR_ARG;
+    if (!w) return set_err(c, SSG_ERR_ARG, "null pointer");
+    int rc = ma_check_params(c, p);
+    if (rc) return rc;
+    const int64_t need = (int64_t)c->nx * c->ny * c->nz;
+    if (n != need) return set_err(c, SSG_ERR_ARG, "size %lld != nx*ny*nz = %lld", (long long)n, (long long)need);
+    HIP_OK(c, hipSetDevice(c->device));
+    ssg_ma_diag d{};
+    c->ma_phi_current = false;
+    if ((rc = ma_solve_dev(c, c->b, p, &d))) return rc;
+    c->last_diag = d;
+    if (diag) *diag = d;
+    if (!d.converged) return ma_not_converged(c, d);
+    if ((rc = omega_alloc(c))) return rc;
+    if ((rc = omega_w(c, p->eps))) return rc;
+    HIP_OK(c, hipMemcpyAsync(w, c->om_w, (size_t)need * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return SSG_OK;
+}
 
 int ssg_set_b(ssg_ctx* c, const double* host, int64_t n, const ssg_ma_params* p, ssg_ma_diag* d) {
     if (!c) return SSG_ERR_ARG;

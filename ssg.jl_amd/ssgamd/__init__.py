@@ -54,6 +54,7 @@ EXPORTED_SYMBOLS = (
     "ssg_ensemble_member", "ssg_ensemble_last_error", "ssg_ensemble_timestep", "ssg_ensemble_tuning",
     "ssg_ma_adjoint", "ssg_adjoint_coupled", "ssg_set_b", "ssg_cfl", "ssg_state_size", "ssg_save_state",
     "ssg_load_state", "ssg_timestep_prepare", "ssg_tendency_noalloc", "ssg_set_vertical_grid",
+    "ssg_vertical_velocity",
 )
 
 MA_BODY = 0      # as wired: b on every level, surface slope 0
@@ -144,6 +145,7 @@ def load_library(path: str | None = None):
                                          ctypes.c_int64]
     lib.ssg_tendency.argtypes = [vp, _dp, ctypes.c_int32]
     lib.ssg_set_vertical_grid.argtypes = [vp, _dp, _dp, ctypes.c_int32]
+    lib.ssg_vertical_velocity.argtypes = [vp, ctypes.POINTER(_MaParams), _dp, ctypes.c_int64, ctypes.POINTER(_MaDiag)]
     lib.ssg_tendency_noalloc.argtypes = [ctypes.POINTER(_DomainDesc), ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
     lib.ssg_jacobian.argtypes = [vp, _dp]
@@ -552,6 +554,19 @@ def set_vertical_grid(domain: Domain, z, dz=None) -> None:
     domain.check(domain.lib.ssg_set_vertical_grid(domain.handle, z.ctypes.data_as(_dp), dzp, int(z.size)))
     domain._z = z.copy()
     domain._dz = dz.copy() if dz is not None else make_vertical_grid(z.size, float(z[-1] - z[0]), "custom", z)[1]
+
+
+def compute_vertical_velocity(fields: Fields, domain: Domain, *, eps: float = 0.1, tol: float = 1e-10,
+                              maxiter: int = 50, variant: int = MA_SURFACE) -> tuple:
+    """``compute_vertical_velocity`` (src/extras/omega_eq.jl:574-652) of the
+    current surface buoyancy: returns (w with shape (nz, ny, nx), the
+    Monge-Ampère solve's diagnostics)."""
+    p = _ma_params(eps, tol, maxiter, variant, 0)
+    d = _MaDiag()
+    w = np.empty((domain.Nz, domain.Ny, domain.Nx))
+    domain.check(domain.lib.ssg_vertical_velocity(domain.handle, ctypes.byref(p), w.ctypes.data_as(_dp), w.size,
+                                                  ctypes.byref(d)))
+    return w, d.as_dict()
 
 
 def get_phi3d(domain: Domain) -> np.ndarray:

@@ -300,6 +300,68 @@ def test_vertical_grid_rules_and_nonuniform_rows():
     assert np.abs(r).max() > 1e-3
 
 
+def test_omega_divq_closed_form():
+    """-2ε∇·Q (src/extras/omega_eq.jl:326-405) for Φ = cos x on every level and
+    b = cos 2y: ∂x v_g = Φxx = -cos x, the other gradients of u_g, v_g vanish,
+    ∂y b = -2 sin 2y, so Q = (2 cos x sin 2y, 0), ∇·Q = -2 sin x sin 2y and
+    -2ε∇·Q = 4ε sin x sin 2y."""
+    from oracle.omega import divq_h
+    nx, ny, nz, eps = 16, 16, 4, 0.3
+    X, Y = grid(nx, ny)
+    phi = np.stack([np.cos(X)] * nz)
+    f = divq_h(phi, np.cos(2 * Y), eps)
+    expect = 4 * eps * np.sin(X) * np.sin(2 * Y)
+    for k in range(nz):
+        assert np.abs(np.fft.irfft2(f[k], s=(ny, nx)) - expect).max() < 1e-13
+
+
+def test_omega_vertical_solve_converges_to_closed_form():
+    """(∂zz - k²) w = f with w = 0 at both ends: for f = sin(πz) on z ∈ [0, 1]
+    the solution is -sin(πz)/(k² + π²); the three-point solve converges to it
+    at second order (error ratio 4 per halving of dz), on a uniform grid and
+    with the stencil weights of a stretched one."""
+    from oracle.omega import vertical_solve
+    k2 = np.array([0.0, 4.0, 25.0])
+    errs = []
+    for nz in (17, 33, 65):
+        z = np.linspace(0, 1, nz)
+        dz = np.full(nz, 1.0 / (nz - 1))
+        f = np.sin(np.pi * z)[:, None] * np.ones(3)[None, :]
+        w = vertical_solve(f, dz, k2)
+        exact = -np.sin(np.pi * z)[:, None] / (k2[None, :] + np.pi ** 2)
+        errs.append(np.abs(w - exact).max())
+    assert 3.8 < errs[0] / errs[1] < 4.2 and 3.8 < errs[1] / errs[2] < 4.2
+    # stretched grid given its true spacings: still consistent (error falls with nz)
+    e2 = []
+    for nz in (33, 129):
+        s = np.linspace(0, 1, nz)
+        z = 1 + np.tanh(1.5 * (s - 1)) / np.tanh(1.5)
+        dz = np.append(np.diff(z), z[-1] - z[-2])
+        f = np.sin(np.pi * z)[:, None] * np.ones(1)[None, :]
+        w = vertical_solve(f, dz, np.array([4.0]))
+        e2.append(np.abs(w[:, 0] + np.sin(np.pi * z) / (4 + np.pi ** 2)).max())
+    assert e2[1] < e2[0] / 5
+
+
+def test_omega_reference_integrals_do_not_solve_the_equation():
+    """The vertical integrals of src/extras/omega_eq.jl:408-521 (the path this
+    restatement replaces) leave w(-H) != 0 and (∂zz - k²) w != f."""
+    k, nz = 2.0, 2001
+    z = np.linspace(-1.0, 0.0, nz)
+    f = np.sin(np.pi * (z + 1.0)) + 0.3 * z
+    tr = lambda x, y: np.trapezoid(y, x)
+    i1 = np.array([-tr(z[i:], np.exp(-k * z[i:]) * f[i:] / (2 * k)) * np.exp(k * z[i]) if i < nz - 1 else 0.0
+                   for i in range(nz)])
+    i2 = np.array([-tr(z[i:], np.exp(k * z[i:]) * f[i:] / (2 * k)) * np.exp(-k * z[i]) if i < nz - 1 else 0.0
+                   for i in range(nz)])
+    H = 1.0
+    i2v = -np.exp(-k * H) * tr(z, np.exp(-k * z) * f / (2 * k)) + np.exp(k * H) * tr(z, np.exp(k * z) * f / (2 * k))
+    w = i1 + i2 + 2 * i2v / (-2 * np.sinh(k * H)) * np.sinh(k * z)
+    dz = z[1] - z[0]
+    res = (w[2:] - 2 * w[1:-1] + w[:-2]) / dz ** 2 - k * k * w[1:-1] - f[1:-1]
+    assert abs(w[0]) > 0.05 and np.abs(res).max() > 1.0
+
+
 def test_parsevalsum2_reference_weights():
     """src/transforms.jl:230-274: weights 1 at x-index 0 and Nx/2, 2 for 0 < i < Nx/2, 0 above."""
     nx, ny = 8, 8
