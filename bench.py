@@ -160,8 +160,15 @@ def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2, step_tol=1e-10
     for k, nm in names.items():
         ms, n, by = d.profile_read(k)
         if n:
-            kernels[nm] = {"avg_us": 1e3 * ms / n, "launches": n, "GBps": by / max(ms, 1e-12) / 1e6}
+            kernels[nm] = {"avg_us": 1e3 * ms / n, "launches": n, "GBps": by / max(ms, 1e-12) / 1e6,
+                           "bytes_per_launch": by / n}
     d.profile(False)
+    # roofline of the coupled step's dominant kernel class (the largest share
+    # of the solve's kernel time): algorithmic bytes per launch / duration
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+    kd = kernels[dom]
+    roof = {"bound": "hbm", "kernel": dom, "achieved": kd["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": kd["GBps"] / HBM_PEAK_GBS, "bytes_per_launch": kd["bytes_per_launch"], "avg_launch_us": kd["avg_us"]}
     # restart from the cold state so both precisions step the same trajectory
     f.bs = b0
     ssg.solve_monge_ampere(f, d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY)
@@ -182,7 +189,8 @@ def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2, step_tol=1e-10
     return {"steps_per_s": steps / wall, "steps": steps, "variant": "body (as wired)", "eps": 0.1, "tol": step_tol,
             "cold_solve": cold, "warm_start": warm, "warm_solve_last_iterations": last.iterations,
             "warm_solve_last_iterations_fp32": last.iterations_fp32,
-            "warm_solve_last_relative_residual": last.relative_residual, "ma_kernels": kernels}, fields
+            "warm_solve_last_relative_residual": last.relative_residual, "ma_kernels": kernels,
+            "roofline": roof}, fields
 
 
 def rel_l2(a, b):
@@ -511,6 +519,8 @@ def main():
         c32m, f32m = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP32_MIXED,
                                      args.ma_warm, step_tol=1e-8)
         out["coupled_fp32_tol1e-8"] = c32m
+        out["coupled_fp32_tol1e-8_over_fp64"] = c32m["steps_per_s"] / c64["steps_per_s"]
+        out["roofline_coupled"] = dict(c64["roofline"], form="fp64")
         out["coupled_fp32_tol1e-8_errors"] = {"b_rel_l2": rel_l2(f32m["b"], f64["b"]),
                                               "phis_rel_l2": rel_l2(f32m["phis"], f64["phis"]),
                                               "ke_surface_rel": abs(f32m["ke"] - f64["ke"]) / abs(f64["ke"])}

@@ -89,10 +89,13 @@ template <typename R, int DIR> struct SmallDFT<R, 1, DIR> {
 
 __host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 
-// LDS index padding: one spare slot every 16 entries keeps the stride-E
-// writes of the first stage off a single bank group.
-__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
-__host__ __device__ constexpr int lds_line_elems(int n) { return n + n / 16 + 1; }
+// LDS index padding: one spare slot every 2^SSG_LDS_PAD_SHIFT entries keeps
+// the stride-E writes of the first stage off a single bank group.
+#ifndef SSG_LDS_PAD_SHIFT
+#define SSG_LDS_PAD_SHIFT 4
+#endif
+__device__ __forceinline__ int lpad(int i) { return i + (i >> SSG_LDS_PAD_SHIFT); }
+__host__ __device__ constexpr int lds_line_elems(int n) { return n + (n >> SSG_LDS_PAD_SHIFT) + 1; }
 
 // number of twiddle factors a thread applies in the stages after the first,
 // and the offset of stage NS within that list

@@ -30,7 +30,9 @@ from dataclasses import dataclass
 import numpy as np
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.normpath(os.path.join(_PKG, "..", "lib", "libssg_hip.so"))
+# SSG_LIB selects another build of the library (e.g. one compiled with
+# different launch constants) for side-by-side measurement
+LIB_PATH = os.environ.get("SSG_LIB") or os.path.normpath(os.path.join(_PKG, "..", "lib", "libssg_hip.so"))
 
 SSG_OK = 0
 SSG_ERR_ARG = -1
