@@ -233,6 +233,12 @@ int step_graph(ssg_ctx* c, const ssg_time_params* p, int phase, int len, int64_t
     e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return set_err(c, SSG_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    // upload now, so the first replay costs what every later one does
+    e = hipGraphUpload(ex, c->stream);
+    if (e != hipSuccess) {
+        (void)hipGraphExecDestroy(ex);
+        return set_err(c, SSG_ERR_HIP, "hipGraphUpload: %s", hipGetErrorString(e));
+    }
     gc.exec.emplace(key, ex);
     *out = ex;
     return SSG_OK;
