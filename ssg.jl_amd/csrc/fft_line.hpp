@@ -122,15 +122,23 @@ __device__ __forceinline__ void line_sync() {
 }
 
 // Stockham line FFT.  v[] holds x[t + T*m] on entry and X[t + T*m] on exit.
-// `buf` is this line's LDS buffer (lds_line_elems(N) entries).  The twiddle
+// `buf` is this line's LDS buffer (lds_line_elems(N) entries).  With PPREQ
+// and lines wider than a wavefront, the stages alternate between buf and a
+// second buffer `alt` entries further on (ping-pong): a stage reads one
+// buffer and writes the other, so the barrier that keeps a stage's writes
+// behind every thread's reads of the same buffer goes, one block barrier
+// per stage instead of two.  The caller then provides LDS for both.  The
+// twiddle
 // factors a thread needs depend only on t, so they are fetched once per
 // kernel into registers (Twiddles::load, from tw[m] = exp(-2*pi*i*m/N)) and
 // shared by every forward and inverse transform of the kernel; loading them
 // up front keeps global-memory latency out of the barrier-separated stages.
 // Every thread of the block must call run() (it contains block barriers).
-template <typename R, int N, int E> struct LineFFT {
+template <typename R, int N, int E, bool PPREQ = false> struct LineFFT {
     static constexpr int T = N / E;
     static constexpr int NTW = tw_count_from(N, E, E);
+    // ping-pong only where stages are separated by block barriers
+    static constexpr bool PP = PPREQ && (T > 64);
     using C = cpx<R>;
 
     struct Twiddles {
@@ -156,21 +164,23 @@ template <typename R, int N, int E> struct LineFFT {
         }
     };
 
-    template <int DIR, int NS>
-    __device__ __forceinline__ static void stage(C* v, C* buf, int t, const Twiddles& tw) {
+    template <int DIR, int NS, bool PAR = false>
+    __device__ __forceinline__ static void stage(C* v, C* buf, int t, const Twiddles& tw, int alt) {
         constexpr int RAD = cmin(E, N / NS);
         constexpr int Q = E / RAD;
         constexpr bool LAST = (NS * RAD == N);
         constexpr int OFF = tw_offset(N, E, NS);
+        const C* src = (PP && PAR) ? buf + alt : buf;
+        C* dst = PP ? (PAR ? buf : buf + alt) : buf;
         C u[E];
         line_sync<T>();
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int j = t + T * q;
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) u[q * RAD + r] = buf[lpad(j + r * (N / RAD))];
+            for (int r = 0; r < RAD; ++r) u[q * RAD + r] = src[lpad(j + r * (N / RAD))];
         }
-        line_sync<T>();
+        if constexpr (!PP) line_sync<T>();
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int j = t + T * q;
@@ -189,21 +199,24 @@ template <typename R, int N, int E> struct LineFFT {
             } else {
                 const int base = (j / NS) * NS * RAD + k;
 #pragma unroll
-                for (int r = 0; r < RAD; ++r) buf[lpad(base + r * NS)] = w[r];
+                for (int r = 0; r < RAD; ++r) dst[lpad(base + r * NS)] = w[r];
             }
         }
-        if constexpr (!LAST) stage<DIR, NS * RAD>(v, buf, t, tw);
+        if constexpr (!LAST) stage<DIR, NS * RAD, !PAR>(v, buf, t, tw, alt);
     }
 
     // NB independent lines of the same thread group transformed in lockstep:
     // one barrier pair per stage for all of them, NB-fold independent work
     // between barriers.
-    template <int DIR, int NS, int NB>
-    __device__ __forceinline__ static void stage_n(C (&v)[NB][E], C* const (&buf)[NB], int t, const Twiddles& tw) {
+    template <int DIR, int NS, int NB, bool PAR = false>
+    __device__ __forceinline__ static void stage_n(C (&v)[NB][E], C* const (&buf)[NB], int t, const Twiddles& tw,
+                                                   int alt) {
         constexpr int RAD = cmin(E, N / NS);
         constexpr int Q = E / RAD;
         constexpr bool LAST = (NS * RAD == N);
         constexpr int OFF = tw_offset(N, E, NS);
+        const int soff = (PP && PAR) ? alt : 0;
+        const int doff = PP ? (PAR ? 0 : alt) : 0;
         C u[NB][E];
         line_sync<T>();
 #pragma unroll
@@ -212,9 +225,9 @@ template <typename R, int N, int E> struct LineFFT {
             for (int q = 0; q < Q; ++q) {
                 const int j = t + T * q;
 #pragma unroll
-                for (int r = 0; r < RAD; ++r) u[b][q * RAD + r] = buf[b][lpad(j + r * (N / RAD))];
+                for (int r = 0; r < RAD; ++r) u[b][q * RAD + r] = buf[b][soff + lpad(j + r * (N / RAD))];
             }
-        line_sync<T>();
+        if constexpr (!PP) line_sync<T>();
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int j = t + T * q;
@@ -235,15 +248,18 @@ template <typename R, int N, int E> struct LineFFT {
                 } else {
                     const int base = (j / NS) * NS * RAD + k;
 #pragma unroll
-                    for (int r = 0; r < RAD; ++r) buf[b][lpad(base + r * NS)] = w[r];
+                    for (int r = 0; r < RAD; ++r) buf[b][doff + lpad(base + r * NS)] = w[r];
                 }
             }
         }
-        if constexpr (!LAST) stage_n<DIR, NS * RAD, NB>(v, buf, t, tw);
+        if constexpr (!LAST) stage_n<DIR, NS * RAD, NB, !PAR>(v, buf, t, tw, alt);
     }
 
+    // `alt`: offset (in elements) of each buffer's ping-pong partner; read
+    // only when PP
     template <int DIR, int NB>
-    __device__ __forceinline__ static void run_n(C (&v)[NB][E], C* const (&buf)[NB], int t, const Twiddles& tw) {
+    __device__ __forceinline__ static void run_n(C (&v)[NB][E], C* const (&buf)[NB], int t, const Twiddles& tw,
+                                                 int alt = 0) {
 #pragma unroll
         for (int b = 0; b < NB; ++b) SmallDFT<R, E, DIR>::run(v[b]);
         if constexpr (T > 1) {
@@ -252,41 +268,41 @@ template <typename R, int N, int E> struct LineFFT {
             for (int b = 0; b < NB; ++b)
 #pragma unroll
                 for (int r = 0; r < E; ++r) buf[b][lpad(t * E + r)] = v[b][r];
-            stage_n<DIR, E, NB>(v, buf, t, tw);
+            stage_n<DIR, E, NB>(v, buf, t, tw, alt);
         }
     }
 
     // NB lines whose inputs sit in LDS in natural order
     template <int DIR, int NB>
     __device__ __forceinline__ static void run_n_from_lds(C (&v)[NB][E], C* const (&buf)[NB], int t,
-                                                          const Twiddles& tw) {
+                                                          const Twiddles& tw, int alt = 0) {
         __syncthreads();
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int m = 0; m < E; ++m) v[b][m] = buf[b][lpad(t + T * m)];
-        run_n<DIR, NB>(v, buf, t, tw);
+        run_n<DIR, NB>(v, buf, t, tw, alt);
     }
 
     // Transform from registers.
     template <int DIR>
-    __device__ __forceinline__ static void run(C* v, C* buf, int t, const Twiddles& tw) {
+    __device__ __forceinline__ static void run(C* v, C* buf, int t, const Twiddles& tw, int alt = 0) {
         SmallDFT<R, E, DIR>::run(v);
         if constexpr (T > 1) {
             line_sync<T>();  // earlier readers of buf are done
 #pragma unroll
             for (int r = 0; r < E; ++r) buf[lpad(t * E + r)] = v[r];
-            stage<DIR, E>(v, buf, t, tw);
+            stage<DIR, E>(v, buf, t, tw, alt);
         }
     }
 
     // Transform whose input sits in LDS in natural order (buf[lpad(i)] = x[i]).
     template <int DIR>
-    __device__ __forceinline__ static void run_from_lds(C* v, C* buf, int t, const Twiddles& tw) {
+    __device__ __forceinline__ static void run_from_lds(C* v, C* buf, int t, const Twiddles& tw, int alt = 0) {
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = buf[lpad(t + T * m)];
-        run<DIR>(v, buf, t, tw);
+        run<DIR>(v, buf, t, tw, alt);
     }
 
     // Leave the result in LDS in natural order.

@@ -27,6 +27,23 @@ namespace ssg_impl {
 
 constexpr int kMaxLdsBytes = 64 * 1024;
 
+// Ping-pong line buffers for the tendency, filter and transform passes
+// (LineFFT's PPREQ): one block barrier per FFT stage instead of two, for
+// twice the LDS.  Used where a line spans several wavefronts and the doubled
+// LDS of the widest block (256 threads) stays within kMaxPingPongLds, so two
+// blocks still share a CU.  The arithmetic is unchanged: results are
+// bitwise the same either way.  Build with -DSSG_FFT_PINGPONG=0 to turn it
+// off.
+#ifndef SSG_FFT_PINGPONG
+#define SSG_FFT_PINGPONG 1
+#endif
+constexpr int kMaxPingPongLds = 80 * 1024;
+template <typename R, int N, int E, int NBUF>
+constexpr bool pingpong_fits() {
+    return SSG_FFT_PINGPONG && (N / E) > 64 &&
+           2L * (256 / (N / E)) * NBUF * lds_line_elems(N) * (long)sizeof(cpx<R>) <= kMaxPingPongLds;
+}
+
 template <typename R>
 struct PathBuffers {
     cpx<R>* w = nullptr;    // (u, v)
@@ -294,14 +311,15 @@ int launch_rows_fwd(ssg_ctx* c, const S* f, const cpx<R>* w, cpx<R>* PA, cpx<R>*
         const int G = lines_per_block<R>(N, E, rows / rows_per_line);
         const int nblk = rows / (rows_per_line * G);
         RowsFwdArgs<R, S> a{f, w, PA, PB, ncols, G, nblk, B.twx, grid ? *grid : c->g};
-        const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<R>);
+        constexpr bool PP = pingpong_fits<R, N, E, 1>();
+        const size_t lds = (PP ? 2 : 1) * (size_t)G * lds_line_elems(N) * sizeof(cpx<R>);
         const double n2 = (double)c->nx * rows, plane = (double)ncols * rows * sizeof(cpx<R>);
         const double bytes = SRC == SRC_FLUX ? n2 * (sizeof(double) + sizeof(cpx<R>)) + 2 * plane
                              : SRC == SRC_W  ? n2 * sizeof(cpx<R>) + 2 * plane
                                              : n2 * sizeof(S) + plane;
         int slot, rc;
         if ((rc = prof_begin(c, &slot))) return rc;
-        launch_on(c, slot, k_rows_fwd<R, S, N, E, SRC>, dim3(nblk), dim3(G * T), lds, a);
+        launch_on(c, slot, k_rows_fwd<R, S, N, E, SRC, PP>, dim3(nblk), dim3(G * T), lds, a);
         c->launches++;
         HIP_OK(c, hipGetLastError());
         return prof_end(c, slot, 0, bytes);
@@ -331,7 +349,8 @@ int launch_cols_args(ssg_ctx* c, ColsArgs<R> a, int nlines, int nlev) {
         a.nlev = nlev;
         if ((OP == OP_MA_DIFF || OP == OP_MA_ADJ) && nblk > c->ma_partial_cap)
             return set_err(c, SSG_ERR_ARG, "reduction buffer too small (%d blocks)", nblk);
-        const size_t lds = (size_t)G * le;
+        constexpr bool PP = pingpong_fits<R, M, E, cols_nbuf(OP)>();
+        const size_t lds = (PP ? 2 : 1) * (size_t)G * le;
         const int nin = OP == OP_MA_ADJ ? 3
                         : (OP == OP_TEND || OP == OP_TEND_Q || OP == OP_JAC || OP == OP_MA_DIFF || OP == OP_PHIBAR) ? 2
                                                                                                                   : 1;
@@ -339,7 +358,7 @@ int launch_cols_args(ssg_ctx* c, ColsArgs<R> a, int nlines, int nlev) {
         const double bytes = (double)(nin + nout) * nlines * c->ny * sizeof(cpx<R>);
         int slot, rc;
         if ((rc = prof_begin(c, &slot))) return rc;
-        launch_on(c, slot, k_cols<R, M, E, OP>, dim3(nblk), dim3(G * T), lds, a);
+        launch_on(c, slot, k_cols<R, M, E, OP, PP>, dim3(nblk), dim3(G * T), lds, a);
         c->launches++;
         HIP_OK(c, hipGetLastError());
         if (OP == OP_MA_DIFF || OP == OP_MA_ADJ) c->ma_partial_used = nblk;
@@ -379,7 +398,8 @@ int launch_rows_inv(ssg_ctx* c, RowsInvArgs<R> a, int rows = 0, const Grid* grid
         a.nblk = nblk;
         a.twx = B.twx;
         a.g = grid ? *grid : c->g;
-        const size_t lds = (size_t)G * 2 * lds_line_elems(N) * sizeof(cpx<R>);
+        constexpr bool PP = pingpong_fits<R, N, E, 2>();
+        const size_t lds = (PP ? 2 : 1) * (size_t)G * 2 * lds_line_elems(N) * sizeof(cpx<R>);
         const double n2 = (double)c->nx * rows, sc = sizeof(cpx<R>);
         double bytes = (MODE == MODE_PAIR ? 1.0 : 2.0) * a.ncols_in * rows * sc;
         if (EPI == EPI_STORE) bytes += n2 * sizeof(R);
@@ -394,7 +414,7 @@ int launch_rows_inv(ssg_ctx* c, RowsInvArgs<R> a, int rows = 0, const Grid* grid
         if (a.bstage) bytes += n2 * sizeof(double);
         int slot, rc;
         if ((rc = prof_begin(c, &slot))) return rc;
-        launch_on(c, slot, k_rows_inv<R, N, E, MODE, EPI>, dim3(nblk), dim3(G * T), lds, a);
+        launch_on(c, slot, k_rows_inv<R, N, E, MODE, EPI, PP>, dim3(nblk), dim3(G * T), lds, a);
         c->launches++;
         HIP_OK(c, hipGetLastError());
         return prof_end(c, slot, 2, bytes);

@@ -205,10 +205,10 @@ __device__ __forceinline__ void store_split_stride(cpx<R>* lds, int G, int lstri
     }
 }
 
-template <typename R, typename S, int N, int E, int SRC>
+template <typename R, typename S, int N, int E, int SRC, bool PP = false>
 __device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs<R, S>& a, const int blk) {
     using C = cpx<R>;
-    using F = LineFFT<R, N, E>;
+    using F = LineFFT<R, N, E, PP>;
     constexpr int T = F::T;
     constexpr int LE = lds_line_elems(N);
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -246,14 +246,14 @@ __device__ __forceinline__ void rows_fwd_body(const RowsFwdArgs<R, S>& a, const 
         }
     }
     C* buf = lds + l * LE;
-    F::template run<-1>(v, buf, t, tw);
+    F::template run<-1>(v, buf, t, tw, G * LE);  // ping-pong partners after the G line buffers
     F::to_lds(v, buf, t);
     store_split<R, N, E>(lds, G, a.ncols, a.PA, SRC == SRC_PAIR ? a.PA : a.PB, a.g.pitch, rowA0, rowB0);
 }
 
-template <typename R, typename S, int N, int E, int SRC>
+template <typename R, typename S, int N, int E, int SRC, bool PP>
 __global__ __launch_bounds__(256) void k_rows_fwd(RowsFwdArgs<R, S> a) {
-    rows_fwd_body<R, S, N, E, SRC>(a, xcd_block(blockIdx.x, a.nblk));
+    rows_fwd_body<R, S, N, E, SRC, PP>(a, xcd_block(blockIdx.x, a.nblk));
 }
 
 // Access to the Monge-Ampère planes.  With `nt` set (planes larger than the
@@ -342,13 +342,14 @@ __host__ __device__ constexpr int cols_nbuf(int op) {
                       : 1);
 }
 
-template <typename R, int M, int E, int OP>
+template <typename R, int M, int E, int OP, bool PP = false>
 __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) {
     using C = cpx<R>;
-    using F = LineFFT<R, M, E>;
+    using F = LineFFT<R, M, E, PP>;
     constexpr int T = F::T;
     constexpr int LE = lds_line_elems(M);
     constexpr int NBUF = cols_nbuf(OP);
+    const int alt = a.G * NBUF * LE;  // ping-pong partners after the block's line buffers
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
     const int l = threadIdx.x / T, t = threadIdx.x - l * T;
@@ -405,7 +406,7 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
             w3[2][m] = live ? srcC[t + T * m] : C{0, 0};
         }
         C* const b3[3] = {lds + (l * NBUF) * LE, lds + (l * NBUF + 1) * LE, lds + (l * NBUF + 2) * LE};
-        F::template run_n<-1, 3>(w3, b3, t, tw);
+        F::template run_n<-1, 3>(w3, b3, t, tw, alt);
         const int lev = a.nlev > 1 ? line - c * a.nlev : 0;
         const bool interior = live && lev >= 1 && lev <= a.nlev - 2;
         const bool top = live && lev == a.nlev - 1;
@@ -440,7 +441,7 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
         // without the x- and y-Nyquist bins (where the transform conventions of
         // that file and of the step path disagree)
         C* const b2[2] = {lds + (l * NBUF) * LE, lds + (l * NBUF + 1) * LE};
-        F::template run_n<-1, 2>(vv, b2, t, tw);
+        F::template run_n<-1, 2>(vv, b2, t, tw, alt);
         const R f = (R)(-2.0 * a.eps);
         const bool xnyq = 2 * c == a.g.nx;
 #pragma unroll
@@ -456,7 +457,7 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
         // φ̄ = Dy U - Dx V (u = -Dy φ, v = Dx φ; Dx, Dy are antisymmetric),
         // left spectral: F(φ̄) = i sdy F(U) - i sdx F(V)
         C* const b2[2] = {lds + (l * NBUF) * LE, lds + (l * NBUF + 1) * LE};
-        F::template run_n<-1, 2>(vv, b2, t, tw);
+        F::template run_n<-1, 2>(vv, b2, t, tw, alt);
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int s = signed_k(t + T * m, a.g.ny);
@@ -476,13 +477,13 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
             w3[2][m] = cscale(p, sym_dxy<R>(a.g, c, s));
         }
         C* const b3[3] = {lds + (l * NBUF) * LE, lds + (l * NBUF + 1) * LE, lds + (l * NBUF + 2) * LE};
-        F::template run_n<+1, 3>(w3, b3, t, tw);
+        F::template run_n<+1, 3>(w3, b3, t, tw, alt);
         store(a.outA, w3[0]);
         store(a.outB, w3[1]);
         store(a.outC, w3[2]);
         return;
     } else if constexpr (OP == OP_INV) {
-        F::template run<+1>(va, buf1[0], t, tw);
+        F::template run<+1>(va, buf1[0], t, tw, alt);
 #pragma unroll
         for (int m = 0; m < E; ++m) va[m] = cscale(va[m], a.scale);
         store(a.outA, va);
@@ -492,9 +493,9 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
         // forward y-transform(s)
         if constexpr (OP != OP_VEL_SPEC) {
             if constexpr (TWO_IN) {
-                F::template run_n<-1, 2>(vv, b2, t, tw);
+                F::template run_n<-1, 2>(vv, b2, t, tw, alt);
             } else {
-                F::template run<-1>(va, buf1[0], t, tw);
+                F::template run<-1>(va, buf1[0], t, tw, alt);
             }
         }
         if constexpr (OP == OP_FWD) {
@@ -542,7 +543,7 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
                 vb[m] = cmul_i(p, sym_dy<R>(a.g, s));
                 va[m] = cmul_i(p, sym_dx<R>(a.g, c, s));
             }
-            F::template run_n<+1, 2>(vv, b2, t, tw);
+            F::template run_n<+1, 2>(vv, b2, t, tw, alt);
             store(a.outA, va);
             store(a.outB, vb);
             return;
@@ -554,7 +555,7 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
                 vb[m] = cmul_i(va[m], dx * a.scale);   // V = Dx P
                 va[m] = cmul_i(va[m], -dy * a.scale);  // U = -Dy P
             }
-            F::template run_n<+1, 2>(vv, b2, t, tw);
+            F::template run_n<+1, 2>(vv, b2, t, tw, alt);
             store(a.outA, va);
             store(a.outB, vb);
             return;
@@ -590,15 +591,15 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
                     va[m] = cscale(va[m], mk * a.scale);
                 }
             }
-            F::template run<+1>(va, buf1[0], t, tw);
+            F::template run<+1>(va, buf1[0], t, tw, alt);
             store(a.outA, va);
         }
     }
 }
 
-template <typename R, int M, int E, int OP>
+template <typename R, int M, int E, int OP, bool PP>
 __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
-    cols_body<R, M, E, OP>(a, blockIdx.x);
+    cols_body<R, M, E, OP, PP>(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -639,10 +640,10 @@ struct RowsInvArgs {
     Grid g;
 };
 
-template <typename R, int N, int E, int MODE, int EPI>
+template <typename R, int N, int E, int MODE, int EPI, bool PP = false>
 __device__ __forceinline__ void rows_inv_body(const RowsInvArgs<R>& a, const int blk) {
     using C = cpx<R>;
-    using F = LineFFT<R, N, E>;
+    using F = LineFFT<R, N, E, PP>;
     constexpr int T = F::T;
     constexpr int LE = lds_line_elems(N);
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -737,7 +738,7 @@ __device__ __forceinline__ void rows_inv_body(const RowsInvArgs<R>& a, const int
     }
     C* buf = lds + l * LE;
     C v[E];
-    F::template run_from_lds<+1>(v, buf, t, tw);
+    F::template run_from_lds<+1>(v, buf, t, tw, 2 * G * LE);
 
     // epilogue
     R sA[E], sB[E];
@@ -833,7 +834,7 @@ __device__ __forceinline__ void rows_inv_body(const RowsInvArgs<R>& a, const int
         vv[1][m] = C{pwB[m].x * sB[m], pwB[m].y * sB[m]};
     }
     C* const bb[2] = {lds + l * LE, lds + (G + l) * LE};
-    F::template run_n<-1, 2>(vv, bb, t, tw);
+    F::template run_n<-1, 2>(vv, bb, t, tw, 2 * G * LE);
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -844,9 +845,9 @@ __device__ __forceinline__ void rows_inv_body(const RowsInvArgs<R>& a, const int
     store_split<R, N, E>(lds, 2 * G, a.ncols_out, a.Fx, a.Fy, pitch, y0, y0);
 }
 
-template <typename R, int N, int E, int MODE, int EPI>
+template <typename R, int N, int E, int MODE, int EPI, bool PP>
 __global__ __launch_bounds__(256) void k_rows_inv(RowsInvArgs<R> a) {
-    rows_inv_body<R, N, E, MODE, EPI>(a, xcd_block(blockIdx.x, a.nblk));
+    rows_inv_body<R, N, E, MODE, EPI, PP>(a, xcd_block(blockIdx.x, a.nblk));
 }
 
 // ---------------------------------------------------------------------------
