@@ -219,12 +219,13 @@ def cpu_c1(inputs, steps):
             "b_rel_l2_gpu_vs_cpu": rel_l2(b_gpu, b)}
 
 
-def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2, step_tol=1e-10):
+def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2, step_tol=1e-10, stage_fp32=False):
     """Coupled step (Monge-Ampère solve at every RK stage, body-forced
-    variant, ε = 0.1, relative residual 1e-10): steps/s plus the Monge-Ampère
-    solve's iterations, time and per-kernel-class profile, and the final
-    surface fields (b, φₛ) and surface kinetic energy for the cross-precision
-    comparison."""
+    variant, ε = 0.1, relative residual `step_tol`; with `stage_fp32` the
+    solves inside a call run in FP32 and the call's last solve in FP64):
+    steps/s, the cold solve's iterations and time, a per-kernel-class profile
+    of 10 further coupled steps (one call), and the final surface fields (b,
+    φₛ) and surface kinetic energy for the cross-precision comparison."""
     import ctypes
     d = ssg.make_domain(nx, ny, nz, precision=prec)
     f = ssg.allocate_fields(d)
@@ -233,30 +234,7 @@ def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2, step_tol=1e-10
     cold = {"iterations": diag["iterations"], "iterations_fp32": diag["iterations_fp32"],
             "ms": diag["solve_time"] * 1e3, "relative_residual": diag["relative_residual"],
             "absolute_residual": diag["final_residual"], "converged": diag["converged"]}
-    d.profile(True)
-    ssg.solve_monge_ampere(f, d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY)
-    names = {1: "k_cols (MA ops)", 2: "k_rows_inv", 3: "k_ma_rows", 4: "k_ma_tridiag", 0: "k_rows_fwd"}
-    kernels = {}
-    for k, nm in names.items():
-        ms, n, by = d.profile_read(k)
-        if n:
-            kernels[nm] = {"avg_us": 1e3 * ms / n, "launches": n, "GBps": by / max(ms, 1e-12) / 1e6,
-                           "bytes_per_launch": by / n}
-    d.profile(False)
-    # roofline of the coupled step's dominant kernel: the row pass k_ma_rows
-    # (the column class times two kernels, the derivative and the residual
-    # passes, whose sum exceeds the row pass but each of which is smaller):
-    # algorithmic bytes per launch / duration
-    dom = "k_ma_rows" if "k_ma_rows" in kernels else max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
-    kd = kernels[dom]
-    form = "fp64" if prec == ssg.SSG_FP64 else "fp32"
-    roof = {"bound": "hbm", "kernel": dom, "achieved": kd["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": kd["GBps"] / HBM_PEAK_GBS, "traffic": load_traffic(f"{dom}:{nx}x{nz}:{form}"),
-            "bytes_per_launch": kd["bytes_per_launch"], "avg_launch_us": kd["avg_us"]}
-    # restart from the cold state so both precisions step the same trajectory
-    f.bs = b0
-    ssg.solve_monge_ampere(f, d, eps=0.1, tol=1e-10, maxiter=60, variant=ssg.MA_BODY)
-    ssg.set_coupling(d, eps=0.1, tol=step_tol, maxiter=60, variant=ssg.MA_BODY, warm=warm)
+    ssg.set_coupling(d, eps=0.1, tol=step_tol, maxiter=60, variant=ssg.MA_BODY, warm=warm, stage_fp32=stage_fp32)
     st = ssg.TimeState(0.0, f)
     params = ssg.TimeParams(dt, scheme=ssg.RK3, filter_freq=10, filter_strength=36.0, filter_order=8,
                             filter_cutoff=0.65)
@@ -269,11 +247,32 @@ def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2, step_tol=1e-10
     last = ssg._MaDiag()
     d.lib.ssg_last_ma_diag(d.handle, ctypes.byref(last))
     fields = {"b": f.bs, "phis": f.phis, "ke": ssg.diagnostics(d)["ke_surface"]}
+    # kernel classes of stepping itself, timed per launch
+    d.profile(True)
+    ssg.timestep(f, d, params, st, 10)
+    names = {0: "k_rows_fwd", 1: "k_cols (tendency and MA ops)", 2: "k_rows_inv", 3: "k_ma_rows",
+             4: "k_ma_tridiag", 11: "k_ma_rows (FP32)", 12: "k_ma_tridiag (FP32)"}
+    kernels = {}
+    for k, nm in names.items():
+        ms, n, by = d.profile_read(k)
+        if n:
+            kernels[nm] = {"avg_us": 1e3 * ms / n, "launches": n, "GBps": by / max(ms, 1e-12) / 1e6,
+                           "bytes_per_launch": by / n}
+    d.profile(False)
     d.close()
+    # roofline of the coupled step's dominant kernel, the Monge-Ampère row
+    # pass in the precision the stage solves run in: algorithmic bytes per
+    # launch / duration
+    dom = "k_ma_rows (FP32)" if stage_fp32 else "k_ma_rows"
+    kd = kernels[dom]
+    form = "fp32" if stage_fp32 else "fp64"
+    roof = {"bound": "hbm", "kernel": dom, "achieved": kd["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": kd["GBps"] / HBM_PEAK_GBS, "traffic": load_traffic(f"k_ma_rows:{nx}x{nz}:{form}"),
+            "bytes_per_launch": kd["bytes_per_launch"], "avg_launch_us": kd["avg_us"]}
     return {"steps_per_s": steps / wall, "steps": steps, "variant": "body (as wired)", "eps": 0.1, "tol": step_tol,
-            "cold_solve": cold, "warm_start": warm, "warm_solve_last_iterations": last.iterations,
-            "warm_solve_last_iterations_fp32": last.iterations_fp32,
-            "warm_solve_last_relative_residual": last.relative_residual, "ma_kernels": kernels,
+            "stage_solves": "fp32" if stage_fp32 else ("fp64" if prec == ssg.SSG_FP64 else "fp32 phase, fp64 residual"),
+            "cold_solve": cold, "warm_start": warm, "last_call_final_solve_iterations": last.iterations,
+            "last_call_final_solve_relative_residual": last.relative_residual, "ma_kernels": kernels,
             "roofline": roof}, fields
 
 
@@ -357,8 +356,9 @@ def measure_stiff(ssg, steps, adj_steps, eps=0.25):
     the run proceeds at ε = 0.25 (ε·max(b0) = 0.875).  Measured: the cold
     Monge-Ampère solves, FP32-mixed steps with φₛ frozen at the ε = 0.25
     solution for `steps` steps, FP32-mixed coupled steps (a solve per RK
-    stage) for `adj_steps` steps, and the FP64 checkpointed adjoint of those
-    coupled steps (checkpoints every 10 steps)."""
+    stage, FP32 stage solves) for `adj_steps` steps, and the FP64
+    checkpointed adjoint of those coupled steps (checkpoints every 10
+    steps)."""
     from ssgamd.ics import spectral_b0
     n, nz, dt = 2048, 64, 1e-4
     b0 = spectral_b0(n, n, seed=7, k0=12.0)
@@ -389,7 +389,7 @@ def measure_stiff(ssg, steps, adj_steps, eps=0.25):
     out["frozen_fp32_steps_per_s"] = (steps - 10) / (time.perf_counter() - t0)
     if adj_steps > 0:
         b_start = f.bs
-        ssg.set_coupling(d, eps=eps, tol=1e-10, maxiter=100, variant=ssg.MA_BODY, warm=2)
+        ssg.set_coupling(d, eps=eps, tol=1e-10, maxiter=100, variant=ssg.MA_BODY, warm=2, stage_fp32=True)
         st = ssg.TimeState(0.0, f)
         f.bs = b_start
         t0 = time.perf_counter()
@@ -492,7 +492,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None)
-    ap.add_argument("--coupled-steps", type=int, default=30,
+    ap.add_argument("--coupled-steps", type=int, default=200,
                     help="also time this many coupled steps (Monge-Ampère solve per stage); 0 = skip")
     ap.add_argument("--ma-warm", type=int, default=2, choices=(0, 1, 2),
                     help="warm start of the coupled step's solves: 0 cold, 1 last D(Φ), 2 extrapolated")
@@ -597,25 +597,26 @@ def main():
     if args.coupled_steps > 0 and not args.only:
         log(f"bench: coupled RK3 {nx}x{ny}x{nz}, {args.coupled_steps} steps per precision")
         c64, f64 = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP64, args.ma_warm)
+        # the precision form: FP32 stage solves, FP64 solve for the returned b
         c32, f32 = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP32_MIXED,
-                                   args.ma_warm)
+                                   args.ma_warm, stage_fp32=True)
+        # FP32-mixed tendency with every stage solve checked on an FP64
+        # residual at 1e-10
+        c32s, f32s = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP32_MIXED,
+                                     args.ma_warm)
+
+        def errs(a, ref, nsteps):
+            return {"b_rel_l2": rel_l2(a["b"], ref["b"]), "phis_rel_l2": rel_l2(a["phis"], ref["phis"]),
+                    "ke_surface_rel": abs(a["ke"] - ref["ke"]) / abs(ref["ke"]), "steps": nsteps}
         out["coupled_fp64"] = c64
         out["coupled_fp32"] = c32
         out["coupled_fp32_over_fp64"] = c32["steps_per_s"] / c64["steps_per_s"]
-        out["coupled_fp32_errors"] = {"b_rel_l2": rel_l2(f32["b"], f64["b"]),
-                                      "phis_rel_l2": rel_l2(f32["phis"], f64["phis"]),
-                                      "ke_surface_rel": abs(f32["ke"] - f64["ke"]) / abs(f64["ke"]),
-                                      "steps": args.coupled_steps + 1}
-        # the FP32-mixed step with its stage solves at a tolerance matched to
-        # the FP32 tendency's own rounding, against the FP64 step at 1e-10
-        c32m, f32m = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_steps, ssg.SSG_FP32_MIXED,
-                                     args.ma_warm, step_tol=1e-8)
-        out["coupled_fp32_tol1e-8"] = c32m
-        out["coupled_fp32_tol1e-8_over_fp64"] = c32m["steps_per_s"] / c64["steps_per_s"]
-        out["roofline_coupled"] = dict(c64["roofline"], form="fp64")
-        out["coupled_fp32_tol1e-8_errors"] = {"b_rel_l2": rel_l2(f32m["b"], f64["b"]),
-                                              "phis_rel_l2": rel_l2(f32m["phis"], f64["phis"]),
-                                              "ke_surface_rel": abs(f32m["ke"] - f64["ke"]) / abs(f64["ke"])}
+        out["coupled_fp32_errors"] = errs(f32, f64, args.coupled_steps + 1)
+        out["coupled_fp32_strict"] = c32s
+        out["coupled_fp32_strict_over_fp64"] = c32s["steps_per_s"] / c64["steps_per_s"]
+        out["coupled_fp32_strict_errors"] = errs(f32s, f64, args.coupled_steps + 1)
+        out["roofline_coupled"] = dict(c32["roofline"], form="fp32 stage solves")
+        out["roofline_coupled_fp64"] = dict(c64["roofline"], form="fp64")
     if args.adjoint_steps > 0 and not args.only:
         log(f"bench: adjoint configuration 256x256x16, {args.adjoint_steps} steps")
         out["adjoint"] = measure_adjoint(ssg, args.adjoint_steps)

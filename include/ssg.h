@@ -114,7 +114,17 @@ typedef struct {
                           last converged solves: in coupled stepping quadratically over the same stage of
                           the last three steps, otherwise 2 D̂_k - D̂_{k-1}.  Writing b or changing the
                           coupling resets that history; a failed warm start is retried cold. */
+    int32_t stage_fp32; /* coupled stepping of an SSG_FP32_MIXED context only (0 elsewhere): 1 runs the
+                          solves inside a ssg_timestep call (the RK stages and the step ends before the
+                          last) entirely in FP32 — transforms, products, D̂, warm-start history — to the
+                          relative residual SSG_MA_STAGE_TOL32, from which the tendency takes (u, v); the
+                          solve for the b the call returns stays FP64 to `tol`, so φₛ, (u, v), the
+                          diagnostics and a saved state after the call are FP64 solves.  No counterpart
+                          in the reference, whose timestep! keeps φₛ frozen. */
 } ssg_ma_params;
+
+/* relative residual of the FP32 stage solves (ssg_ma_params.stage_fp32) */
+#define SSG_MA_STAGE_TOL32 1e-6
 
 /* solve_ssg_equation diagnostics (src/poisson.jl:928-958) */
 typedef struct {
@@ -294,7 +304,9 @@ int ssg_last_timing(ssg_ctx* ctx, double* ms, int64_t* launches);
    measured milliseconds, the launch count and the algorithmic bytes (every
    input read once, every output written once).  Classes: 0 = row pass
    forward (k_rows_fwd), 1 = column pass (k_cols), 2 = row pass inverse with
-   fused stage epilogue (k_rows_inv). */
+   fused stage epilogue (k_rows_inv), 3 = Monge-Ampère row pass (k_ma_rows),
+   4 = Monge-Ampère per-wavenumber solve (k_ma_tridiag); 11 and 12 = the
+   FP32 launches of classes 3 and 4. */
 int ssg_profile(ssg_ctx* ctx, int32_t enable);
 
 /* Autotuned launch shape of the three line-FFT kernel classes for one path

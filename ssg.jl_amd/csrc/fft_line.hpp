@@ -16,6 +16,10 @@ template <typename R> struct cpx_t;
 template <> struct cpx_t<double> { using type = double2; };
 template <> struct cpx_t<float> { using type = float2; };
 template <typename R> using cpx = typename cpx_t<R>::type;
+// the real type of a complex type
+template <typename C> struct cpx_real_t;
+template <> struct cpx_real_t<double2> { using type = double; };
+template <> struct cpx_real_t<float2> { using type = float; };
 
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }

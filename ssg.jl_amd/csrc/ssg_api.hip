@@ -116,9 +116,9 @@ int run_steps_coupled(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, i
         ~PeriodScope() { c->ma_period = 0; }
     } period_scope{c};
     c->ma_period = p->scheme == SSG_RK3 ? 3 : 1;
-    auto solve = [&](const double* bsrc) -> int {
+    auto solve = [&](const double* bsrc, bool final_solve) -> int {
         c->ma_phi_current = false;
-        int r = ma_solve_dev(c, bsrc, &c->ma_par, &d);
+        int r = ma_solve_coupled(c, bsrc, &c->ma_par, &d, final_solve);
         if (r) return r;
         if (!d.converged)
             return ma_not_converged(c, d);
@@ -126,7 +126,7 @@ int run_steps_coupled(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, i
     };
     auto stage_tendency = [&](const double* bsrc, bool solved) -> int {
         int r;
-        if (!solved && (r = solve(bsrc))) return r;
+        if (!solved && (r = solve(bsrc, false))) return r;
         if ((r = flux_forward<R>(c, c->ncols_tend, bsrc))) return r;
         return tendency_cols<R>(c, p->scheme == SSG_AB2_LOWSTORAGE ? p->ab2_reference_aliasing : 0);
     };
@@ -152,7 +152,7 @@ int run_steps_coupled(ssg_ctx* c, const ssg_time_params* p, ssg_time_state* s, i
         s->step = step_after;
         s->dt_actual = dt;
         if (filt && (rc = filter_b<R>(c, p->filter_strength, p->filter_order, p->filter_cutoff, 0))) return rc;
-        if ((rc = solve(c->b))) return rc;
+        if ((rc = solve(c->b, it + 1 == nsteps))) return rc;
         c->ma_phi_current = true;
     }
     return SSG_OK;
@@ -714,7 +714,7 @@ int ssg_save_state(ssg_ctx* c, const ssg_time_state* s, void* buf, size_t bytes)
                                                state_bytes(c));
     HIP_OK(c, hipSetDevice(c->device));
     StateHeader h{};
-    memcpy(h.magic, "SSGSTAT2", 8);
+    memcpy(h.magic, "SSGSTAT3", 8);
     h.nx = c->nx;
     h.ny = c->ny;
     h.nz = c->nz;
@@ -750,8 +750,12 @@ int ssg_load_state(ssg_ctx* c, const void* buf, size_t bytes, ssg_time_state* s)
     StateHeader h{};
     if (bytes < sizeof(h)) return set_err(c, SSG_ERR_ARG, "state buffer too small");
     memcpy(&h, buf, sizeof(h));
-    const bool v1 = memcmp(h.magic, "SSGSTAT1", 8) == 0;
-    if (!v1 && memcmp(h.magic, "SSGSTAT2", 8) != 0) return set_err(c, SSG_ERR_ARG, "not a saved step state");
+    const int version = memcmp(h.magic, "SSGSTAT1", 8) == 0   ? 1
+                        : memcmp(h.magic, "SSGSTAT2", 8) == 0 ? 2
+                        : memcmp(h.magic, "SSGSTAT3", 8) == 0 ? 3
+                                                              : 0;
+    if (!version) return set_err(c, SSG_ERR_ARG, "not a saved step state");
+    const bool v1 = version == 1;
     if (h.nx != c->nx || h.ny != c->ny || h.precision != c->precision)
         return set_err(c, SSG_ERR_ARG, "state of a %dx%d grid (precision %d) does not fit this %dx%d context (%d)",
                        h.nx, h.ny, h.precision, c->nx, c->ny, c->precision);
@@ -775,7 +779,7 @@ int ssg_load_state(ssg_ctx* c, const void* buf, size_t bytes, ssg_time_state* s)
     HIP_OK(c, hipStreamSynchronize(c->stream));
     if (!v1) {
         const size_t used = (size_t)(p - static_cast<const char*>(buf));
-        if ((rc = ma_state_load(c, p, bytes - used))) return rc;
+        if ((rc = ma_state_load(c, p, bytes - used, version))) return rc;
     }
     c->ab2_have_history = h.have_ab2 != 0;
     s->t = h.t;

@@ -66,13 +66,16 @@ using namespace ssg_impl;
 // ring[0] holds the last iterate and the solver's working buffer is scratch;
 // otherwise the working buffer holds it.  Buffers rotate between the ring,
 // the working buffer and the spares; no iterate is ever copied.
-struct SolveHistory {
-    cpx<double>* ring[9] = {};
+template <typename C>
+struct SolveHistoryT {
+    C* ring[9] = {};
     int n = 0;
-    cpx<double>* spare[10] = {};
+    C* spare[10] = {};
     int nspare = 0;
     int period = -1;  // the solves-per-step pattern under which the entries were pushed
 };
+using SolveHistory = SolveHistoryT<cpx<double>>;
+using SolveHistory32 = SolveHistoryT<cpx<float>>;  // FP32 stage solves of the coupled step
 
 struct ssg_ctx {
     int nx = 0, ny = 0, nz = 0;
@@ -122,6 +125,11 @@ struct ssg_ctx {
     // derivative planes alias ma_p (an FP64 plane holds two FP32 planes)
     cpx<float>* ma32_phi = nullptr;
     cpx<float>* ma32_d = nullptr;
+    // FP32 stage solves of the coupled step (ssg_ma_params.stage_fp32): the
+    // history of their D̂ (working buffer: ma32_d), and whether ma32_d or
+    // that history holds a warm start
+    SolveHistory32 ma_hist32;
+    bool ma_have_d32 = false;
     int ma_ept[2] = {0, 0};           // elements per thread of k_ma_rows per path (0 FP64, 1 FP32)
     Grid g3{};                        // 3D plane geometry: pitch = nz * ny
     std::vector<double> dz;           // vertical spacings (Nz entries) of a custom grid; empty: uniform Lz/(Nz-1)
@@ -577,13 +585,14 @@ inline int ma_not_converged(ssg_ctx* c, const ssg_ma_diag& d) {
 extern std::mutex g_tune_mu;
 const char* noalloc_last_error();
 int ma_solve_dev(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_diag* d);
+int ma_solve_coupled(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_diag* d, bool final_solve);
 int rfft2_device(ssg_ctx* c, const double* src, cpx<double>* dst = nullptr);
 int ma_surface_outputs(ssg_ctx* c);
 void ma_free(ssg_ctx* c);
 void ma_history_reset(ssg_ctx* c);
 size_t ma_state_bytes(const ssg_ctx* c);
 int ma_state_save(ssg_ctx* c, char* p);
-int ma_state_load(ssg_ctx* c, const char* p, size_t avail);
+int ma_state_load(ssg_ctx* c, const char* p, size_t avail, int version);
 void adjc_free(ssg_ctx* c);
 int adj_alloc(ssg_ctx* c);
 

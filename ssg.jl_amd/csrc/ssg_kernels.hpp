@@ -1018,19 +1018,23 @@ static __global__ __launch_bounds__(256) void k_wmax(const double2* __restrict__
 // warm-start extrapolation: out = ca a + cb b + cc c (c may be null).  One
 // element per thread (grid of ceil(n / 256) blocks); plane access as in
 // ld_plane / st_plane.
-static __global__ __launch_bounds__(256) void k_extrap(double2* __restrict__ out, const double2* __restrict__ a,
-                                                       double ca, const double2* __restrict__ b, double cb,
-                                                       const double2* __restrict__ c, double cc, long n, int nt) {
+// C = double2 or float2 (FP32 stage solves: FP64 arithmetic, one rounding on
+// the store).
+template <typename C>
+__global__ __launch_bounds__(256) void k_extrap(C* __restrict__ out, const C* __restrict__ a, double ca,
+                                                const C* __restrict__ b, double cb, const C* __restrict__ c,
+                                                double cc, long n, int nt) {
+    using R = typename cpx_real_t<C>::type;
     const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (q >= n) return;
-    const double2 x = ld_plane<double>(a + q, nt), y = ld_plane<double>(b + q, nt);
-    double2 r = {ca * x.x + cb * y.x, ca * x.y + cb * y.y};
+    const C x = ld_plane<R>(a + q, nt), y = ld_plane<R>(b + q, nt);
+    double rx = ca * (double)x.x + cb * (double)y.x, ry = ca * (double)x.y + cb * (double)y.y;
     if (c) {
-        const double2 z = ld_plane<double>(c + q, nt);
-        r.x += cc * z.x;
-        r.y += cc * z.y;
+        const C z = ld_plane<R>(c + q, nt);
+        rx += cc * (double)z.x;
+        ry += cc * (double)z.y;
     }
-    st_plane<double>(out + q, r, nt);
+    st_plane<R>(out + q, C{(R)rx, (R)ry}, nt);
 }
 
 // out = a x + b y (y may be null)

@@ -4,7 +4,8 @@
 
 namespace ssg_impl {
 
-void hist_free(SolveHistory& h);
+template <typename C>
+void hist_free(SolveHistoryT<C>& h);
 
 // ---------------------------------------------------------------------------
 // Monge-Ampère solve (FP64)
@@ -108,6 +109,7 @@ void ma_free(ssg_ctx* c) {
     (void)hipFree(c->ma32_phi);
     (void)hipFree(c->ma32_d);
     hist_free(c->ma_hist);
+    hist_free(c->ma_hist32);
     (void)hipFree(c->ma_coef);
     (void)hipFree(c->ma_partial);
     if (c->ma_sum_host) (void)hipHostFree(c->ma_sum_host);
@@ -124,14 +126,16 @@ int hist_cap(int period) {
     return period == 3 ? 9 : 3;
 }
 
-void hist_free(SolveHistory& h) {
+template <typename C>
+void hist_free(SolveHistoryT<C>& h) {
     for (int i = 0; i < h.n; ++i) (void)hipFree(h.ring[i]);
     for (int i = 0; i < h.nspare; ++i) (void)hipFree(h.spare[i]);
-    h = SolveHistory{};
+    h = SolveHistoryT<C>{};
 }
 
 // Drop the history when the working buffer itself holds the newest iterate.
-void hist_drop(SolveHistory& h) {
+template <typename C>
+void hist_drop(SolveHistoryT<C>& h) {
     for (int i = 0; i < h.n; ++i) {
         h.spare[h.nspare++] = h.ring[i];
         h.ring[i] = nullptr;
@@ -141,7 +145,8 @@ void hist_drop(SolveHistory& h) {
 
 // Drop the history, keeping the last iterate as the working buffer (the
 // plain warm start's invariant).  Pointer moves only: no device work.
-void hist_reset(SolveHistory& h, cpx<double>*& work) {
+template <typename C>
+void hist_reset(SolveHistoryT<C>& h, C*& work) {
     if (h.n == 0) return;
     h.spare[h.nspare++] = work;
     work = h.ring[0];
@@ -155,7 +160,8 @@ void hist_reset(SolveHistory& h, cpx<double>*& work) {
 // buffer.  Entries pushed under another period are dropped first, so the
 // ring only holds consecutive solves of one pattern (their lags then mean
 // the same stage of earlier steps).
-int hist_push(ssg_ctx* c, SolveHistory& h, cpx<double>*& work, int period) {
+template <typename C>
+int hist_push(ssg_ctx* c, SolveHistoryT<C>& h, C*& work, int period) {
     const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
     if (h.period != period) {
         hist_drop(h);
@@ -166,13 +172,13 @@ int hist_push(ssg_ctx* c, SolveHistory& h, cpx<double>*& work, int period) {
         h.spare[h.nspare++] = h.ring[--h.n];
         h.ring[h.n] = nullptr;
     }
-    cpx<double>* freed = nullptr;
+    C* freed = nullptr;
     if (h.n == cap) freed = h.ring[--h.n];
     for (int i = h.n; i > 0; --i) h.ring[i] = h.ring[i - 1];
     h.ring[0] = work;
     h.n++;
     if (!freed && h.nspare > 0) freed = h.spare[--h.nspare];
-    if (!freed && hipMalloc(&freed, c3 * sizeof(cpx<double>)) != hipSuccess) {
+    if (!freed && hipMalloc(&freed, c3 * sizeof(C)) != hipSuccess) {
         // no memory for more history: undo the push and keep the plain warm
         // start (the working buffer keeps the newest iterate)
         (void)hipGetLastError();
@@ -194,18 +200,19 @@ static int ma_stream_planes(const ssg_ctx* c) { return c->ma_stream; }
 // steps, 3 x[-P] - 3 x[-2P] + x[-3P]: its error is O(dt³), and it carries no
 // error from the RK stage states' own O(dt²) offsets from the trajectory.
 // With less history: the linear 2 x[-1] - x[-2], else x[-1].
-int hist_extrapolate(ssg_ctx* c, const SolveHistory& h, cpx<double>* work, int period) {
+template <typename C>
+int hist_extrapolate(ssg_ctx* c, const SolveHistoryT<C>& h, C* work, int period) {
     const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
     const int n = h.n, P = period;
-    cpx<double>* const* r = h.ring;
+    C* const* r = h.ring;
     if (P > 0 && n >= 3 * P)
-        hipLaunchKernelGGL(k_extrap, dim3(grid_all((long)c3)), dim3(256), 0, c->stream, work, r[P - 1], 3.0,
+        hipLaunchKernelGGL(k_extrap<C>, dim3(grid_all((long)c3)), dim3(256), 0, c->stream, work, r[P - 1], 3.0,
                            r[2 * P - 1], -3.0, r[3 * P - 1], 1.0, (long)c3, ma_stream_planes(c));
     else if (n >= 2)
-        hipLaunchKernelGGL(k_extrap, dim3(grid_all((long)c3)), dim3(256), 0, c->stream, work, r[0], 2.0, r[1], -1.0,
-                           (const double2*)nullptr, 0.0, (long)c3, ma_stream_planes(c));
+        hipLaunchKernelGGL(k_extrap<C>, dim3(grid_all((long)c3)), dim3(256), 0, c->stream, work, r[0], 2.0, r[1],
+                           -1.0, (const C*)nullptr, 0.0, (long)c3, ma_stream_planes(c));
     else {
-        HIP_OK(c, hipMemcpyAsync(work, r[0], c3 * sizeof(cpx<double>), hipMemcpyDeviceToDevice, c->stream));
+        HIP_OK(c, hipMemcpyAsync(work, r[0], c3 * sizeof(C), hipMemcpyDeviceToDevice, c->stream));
         return SSG_OK;
     }
     c->launches++;
@@ -218,6 +225,8 @@ int hist_extrapolate(ssg_ctx* c, const SolveHistory& h, cpx<double>* work, int p
 // counts as solved for b by the coupled stepper
 void ma_history_reset(ssg_ctx* c) {
     hist_reset(c->ma_hist, c->ma_dprev);
+    hist_drop(c->ma_hist32);
+    c->ma_have_d32 = false;
     c->ma_phi_current = false;
 }
 
@@ -254,7 +263,7 @@ int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d, const cpx<R>* Dh, 
         launch_on(c, slot, k_ma_tridiag<R, P, 128>, dim3(nblk), dim3(128), 0, a);
     c->launches++;
     HIP_OK(c, hipGetLastError());
-    return prof_end(c, slot, 4, bytes);
+    return prof_end(c, slot, std::is_same<R, float>::value ? 12 : 4, bytes);
 }
 
 // x-inverse of the three derivative planes, D = Φxx Φyy - Φxy², x-FFT of D
@@ -289,7 +298,7 @@ int ma_rows(ssg_ctx* c, cpx<R>* const (&planes)[3]) {
         launch_on(c, slot, k_ma_rows<R, N, E>, dim3(nblk), dim3(G * T), lds, a);
         c->launches++;
         HIP_OK(c, hipGetLastError());
-        return prof_end(c, slot, 3, bytes);
+        return prof_end(c, slot, std::is_same<R, float>::value ? 11 : 3, bytes);
     });
 }
 
@@ -595,6 +604,8 @@ int ma_check_params(ssg_ctx* c, const ssg_ma_params* p) {
     if (p->variant != SSG_MA_BODY && p->variant != SSG_MA_SURFACE)
         return set_err(c, SSG_ERR_ARG, "unknown Monge-Ampere variant %d", p->variant);
     if (p->warm < 0 || p->warm > 2) return set_err(c, SSG_ERR_ARG, "warm must be 0, 1 or 2 (got %d)", p->warm);
+    if (p->stage_fp32 != 0 && p->stage_fp32 != 1)
+        return set_err(c, SSG_ERR_ARG, "stage_fp32 must be 0 or 1 (got %d)", p->stage_fp32);
     return SSG_OK;
 }
 
@@ -667,6 +678,132 @@ int ma_solve_dev(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_di
     return SSG_OK;
 }
 
+// (u, v) = (-∂y, ∂x) of Φ̂'s top level in path precision R, into the path's
+// velocity buffer (the tendency's operand); φₛ itself is not formed.
+template <typename R>
+int ma_surface_w(ssg_ctx* c, const cpx<R>* phi) {
+    PathBuffers<R>& B = bufs<R>(c);
+    ColsArgs<R> a{};
+    a.inA = phi + (size_t)(c->nz - 1) * c->ny;
+    a.in_stride = (long)c->nz * c->ny;
+    a.outA = B.Fx;
+    a.outB = B.Fy;
+    a.scale = (R)(1.0 / ((double)c->nx * c->ny));
+    int rc = launch_cols_args<R, OP_VEL_SPEC>(c, a, c->ncols_full, 1);
+    if (rc) return rc;
+    RowsInvArgs<R> w{};
+    w.P0 = B.Fx;
+    w.P1 = B.Fy;
+    w.ncols_in = c->ncols_full;
+    w.w_out = B.w;
+    return launch_rows_inv<R, MODE_PLANES, EPI_STORE_W>(c, w);
+}
+
+// After an FP64 solve in a run with FP32 stage solves: its D̂, rounded,
+// continues the FP32 history (it takes that solve's place in the stage
+// pattern), and the FP64 history keeps only the newest D̂ as its working
+// buffer (pointer moves).
+static int ma_seed32(ssg_ctx* c, const ssg_ma_params* p) {
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    const cpx<double>* newest = c->ma_hist.n > 0 ? c->ma_hist.ring[0] : c->ma_dprev;
+    int rc = cpx_convert<double, float>(c, newest, c->ma32_d, c3);
+    if (rc) return rc;
+    hist_reset(c->ma_hist, c->ma_dprev);
+    if (p->warm >= 2 && (rc = hist_push(c, c->ma_hist32, c->ma32_d, c->ma_period))) return rc;
+    c->ma_have_d32 = true;
+    return SSG_OK;
+}
+
+// FP32 stage solve (ssg_ma_params.stage_fp32): warm start from the FP32
+// history (extrapolated as in hist_extrapolate), then Picard iterations whose
+// transforms, products, D̂ and residual sums are FP32 (the per-wavenumber
+// solve keeps FP64 arithmetic), to the relative residual
+// max(tol, SSG_MA_STAGE_TOL32).  Leaves (u, v) of the FP32 path; φₛ, the
+// FP64 velocities and the FP64 Φ̂ keep the last FP64 solve.  d->converged = 0
+// when it fails (the caller then solves in FP64).
+static int ma_solve_stage32(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_diag* d) {
+    int rc;
+    if ((rc = rfft2_device(c, bs, c->ma_b2))) return rc;
+    if (c->ma_hist32.n > 0 && (rc = hist_extrapolate(c, c->ma_hist32, c->ma32_d, c->ma_period))) return rc;
+    memset(d, 0, sizeof(*d));
+    const double tol32 = std::max(p->tol, (double)SSG_MA_STAGE_TOL32);
+    cpx<float>* const pl[3] = {(cpx<float>*)c->ma_p[0], (cpx<float>*)c->ma_p[1], (cpx<float>*)c->ma_p[2]};
+    if ((rc = ma_tridiag<float, float>(c, p, 1, c->ma32_d, c->ma32_phi))) return rc;
+    double best = 1e300, ra = 0, rr = 0;
+    for (int it = 0;; ++it) {
+        if ((rc = ma_eval<float>(c, p, c->ma32_phi, pl, c->ma32_d, &ra, &rr))) return rc;
+        d->final_residual = ra;
+        d->relative_residual = rr;
+        if (it < 64) d->history[it] = rr;
+        d->iterations = it;
+        d->iterations_fp32 = it + 1;
+        if (!std::isfinite(rr)) break;
+        best = std::min(best, rr);
+        if (rr > 100.0 * best) break;
+        if (rr <= tol32) {
+            d->converged = 1;
+            break;
+        }
+        if (it >= p->maxiter) break;
+        if ((rc = ma_tridiag<float, float>(c, p, 1, c->ma32_d, c->ma32_phi))) return rc;
+    }
+    if (!d->converged) {
+        // the FP32 history no longer leads anywhere; the FP64 solve that
+        // follows reseeds it
+        hist_drop(c->ma_hist32);
+        c->ma_have_d32 = false;
+        return SSG_OK;
+    }
+    c->ma_last_r0 = d->history[0];
+    if (p->warm >= 2 && (rc = hist_push(c, c->ma_hist32, c->ma32_d, c->ma_period))) return rc;
+    return ma_surface_w<float>(c, c->ma32_phi);
+}
+
+// The coupled stepper's solve for the buoyancy bs.  With FP32 stage solves
+// (FP32-mixed context, params->stage_fp32) every solve but the call's last
+// (final_solve) runs in FP32 once a warm start is held; the others, and an
+// FP32 solve that fails, run in FP64 from the FP32 history's guess and then
+// continue that history.
+// SSG_MA_TRACE=1 in the environment prints one line per coupled solve to
+// stderr: FP32 or FP64, evaluations, first and last relative residual.
+static void ma_trace(const char* kind, const ssg_ma_diag& d) {
+    const char* e = getenv("SSG_MA_TRACE");
+    if (e && e[0] == '1')
+        fprintf(stderr, "ssg_ma_trace %s evals=%d fp32=%d r0=%.3e r=%.3e conv=%d\n", kind, d.iterations + 1,
+                d.iterations_fp32, d.history[0], d.relative_residual, d.converged);
+}
+
+int ma_solve_coupled(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_diag* d, bool final_solve) {
+    const bool s32 = p->stage_fp32 && c->precision == SSG_FP32_MIXED;
+    int rc;
+    if (!s32) {
+        rc = ma_solve_dev(c, bs, p, d);
+        if (!rc) ma_trace("fp64", *d);
+        return rc;
+    }
+    if ((rc = ma_alloc(c))) return rc;
+    if (!final_solve && c->ma_have_d32) {
+        if ((rc = ma_solve_stage32(c, bs, p, d))) return rc;
+        ma_trace("fp32", *d);
+        if (d->converged) {
+            c->last_diag = *d;
+            return SSG_OK;
+        }
+    }
+    if (c->ma_have_d32) {
+        // FP64 solve warm-started from the FP32 history's guess
+        const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+        if (c->ma_hist32.n > 0 && (rc = hist_extrapolate(c, c->ma_hist32, c->ma32_d, c->ma_period))) return rc;
+        hist_reset(c->ma_hist, c->ma_dprev);
+        if ((rc = cpx_convert<float, double>(c, c->ma32_d, c->ma_dprev, c3))) return rc;
+        c->ma_have_d = true;
+    }
+    if ((rc = ma_solve_dev(c, bs, p, d))) return rc;
+    ma_trace(final_solve ? "fp64-final" : "fp64", *d);
+    if (d->converged && (rc = ma_seed32(c, p))) return rc;
+    return SSG_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Monge-Ampère part of a saved step state: the last solve's Φ̂ (the 3D
 // streamfunction of the reference's checkpoint, src/output.jl:376-379) and
@@ -686,14 +823,26 @@ struct MaStateHeader {
     int32_t phis_from_phi;  // φₛ and the velocities came from Φ̂'s top level (not set since)
     double last_r0;
     ssg_ma_params par;
+    // from blob version 3: the warm-start state is the FP32 stage solves'
+    // history (entries widened to FP64, exactly)
+    int32_t ring32;
+    int32_t reserved;
 };
+// the header of version-2 blobs ends before ring32
+constexpr size_t kMaStateHeaderV2 = offsetof(MaStateHeader, ring32);
 
 static size_t ma_c3_bytes(const ssg_ctx* c) { return (size_t)c->ncols_full * c->nz * c->ny * sizeof(cpx<double>); }
+
+// the FP32 stage solves' history is what a coupled run continues from
+static bool ma_state_ring32(const ssg_ctx* c) { return c->ma_ready && c->ma_par.stage_fp32 && c->ma_have_d32; }
 
 static int ma_state_nbuf(const ssg_ctx* c) {
     if (!c->ma_ready) return 0;
     int n = c->ma_have_phi ? 1 : 0;
-    if (c->ma_have_d) n += c->ma_hist.n > 0 ? c->ma_hist.n : 1;
+    if (ma_state_ring32(c))
+        n += c->ma_hist32.n > 0 ? c->ma_hist32.n : 1;
+    else if (c->ma_have_d)
+        n += c->ma_hist.n > 0 ? c->ma_hist.n : 1;
     return n;
 }
 
@@ -705,9 +854,10 @@ int ma_state_save(ssg_ctx* c, char* p) {
     MaStateHeader h{};
     h.nz = c->nz;
     h.have_phi = c->ma_ready && c->ma_have_phi;
-    h.have_d = c->ma_ready && c->ma_have_d;
-    h.ring_n = h.have_d ? c->ma_hist.n : 0;
-    h.ring_period = c->ma_hist.period;
+    h.ring32 = ma_state_ring32(c) ? 1 : 0;
+    h.have_d = h.ring32 || (c->ma_ready && c->ma_have_d);
+    h.ring_n = h.ring32 ? c->ma_hist32.n : (h.have_d ? c->ma_hist.n : 0);
+    h.ring_period = h.ring32 ? c->ma_hist32.period : c->ma_hist.period;
     h.phi_current = c->ma_phi_current;
     h.coupled = c->coupled;
     h.phis_from_phi = c->ma_phis_from_phi;
@@ -721,9 +871,23 @@ int ma_state_save(ssg_ctx* c, char* p) {
         p += c3b;
         return SSG_OK;
     };
+    // an FP32 entry is widened into the scratch plane ma_p[0] first (stream
+    // order keeps each copy ahead of the next widening)
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    auto put32 = [&](const cpx<float>* src) -> int {
+        int r = cpx_convert<float, double>(c, src, c->ma_p[0], c3);
+        return r ? r : put(c->ma_p[0]);
+    };
     int rc;
     if (h.have_phi && (rc = put(c->ma_phi))) return rc;
-    if (h.have_d) {
+    if (h.ring32) {
+        if (h.ring_n > 0) {
+            for (int i = 0; i < h.ring_n; ++i)
+                if ((rc = put32(c->ma_hist32.ring[i]))) return rc;
+        } else if ((rc = put32(c->ma32_d))) {
+            return rc;
+        }
+    } else if (h.have_d) {
         if (h.ring_n > 0) {
             for (int i = 0; i < h.ring_n; ++i)
                 if ((rc = put(c->ma_hist.ring[i]))) return rc;
@@ -734,12 +898,13 @@ int ma_state_save(ssg_ctx* c, char* p) {
     return SSG_OK;
 }
 
-int ma_state_load(ssg_ctx* c, const char* p, size_t avail) {
+int ma_state_load(ssg_ctx* c, const char* p, size_t avail, int version) {
     MaStateHeader h{};
-    if (avail < sizeof(h)) return set_err(c, SSG_ERR_ARG, "state buffer truncated (Monge-Ampere part)");
-    memcpy(&h, p, sizeof(h));
-    p += sizeof(h);
-    avail -= sizeof(h);
+    const size_t hb = version >= 3 ? sizeof(h) : kMaStateHeaderV2;
+    if (avail < hb) return set_err(c, SSG_ERR_ARG, "state buffer truncated (Monge-Ampere part)");
+    memcpy(&h, p, hb);
+    p += hb;
+    avail -= hb;
     ma_history_reset(c);
     c->ma_have_d = false;
     c->ma_have_phi = false;
@@ -760,7 +925,34 @@ int ma_state_load(ssg_ctx* c, const char* p, size_t avail) {
     };
     if (h.have_phi && (rc = get(c->ma_phi))) return rc;
     SolveHistory& hs = c->ma_hist;
-    if (h.have_d) {
+    const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
+    // an FP32 history entry: through the scratch plane ma_p[0], narrowed
+    // (exactly: it was widened from FP32)
+    auto get32 = [&](cpx<float>* dst) -> int {
+        int r = get(c->ma_p[0]);
+        return r ? r : cpx_convert<double, float>(c, c->ma_p[0], dst, c3);
+    };
+    if (h.ring32 && h.have_d) {
+        if (c->precision != SSG_FP32_MIXED)
+            return set_err(c, SSG_ERR_ARG, "state with FP32 stage solves needs an FP32-mixed context");
+        SolveHistory32& h32 = c->ma_hist32;
+        if (h.ring_n > 0) {
+            for (int i = 0; i < h.ring_n; ++i) {
+                cpx<float>* buf = nullptr;
+                if (h32.nspare > 0)
+                    buf = h32.spare[--h32.nspare];
+                else
+                    HIP_OK(c, hipMalloc(&buf, c3 * sizeof(cpx<float>)));
+                h32.ring[i] = buf;
+                h32.n = i + 1;
+                if ((rc = get32(buf))) return rc;
+            }
+            h32.period = h.ring_period;
+        } else if ((rc = get32(c->ma32_d))) {
+            return rc;
+        }
+        c->ma_have_d32 = true;
+    } else if (h.have_d) {
         if (h.ring_n > 0) {
             for (int i = 0; i < h.ring_n; ++i) {
                 cpx<double>* buf = nullptr;
@@ -1646,7 +1838,7 @@ int ssg_set_coupling(ssg_ctx* c, const ssg_ma_params* p) {
     // the same parameters again (e.g. after a restart that restored them)
     // leave the coupled run's history in place
     if (p && c->coupled && p->eps == c->ma_par.eps && p->tol == c->ma_par.tol && p->maxiter == c->ma_par.maxiter &&
-        p->variant == c->ma_par.variant && p->warm == c->ma_par.warm)
+        p->variant == c->ma_par.variant && p->warm == c->ma_par.warm && p->stage_fp32 == c->ma_par.stage_fp32)
         return SSG_OK;
     ma_history_reset(c);
     if (!p) {
@@ -1655,6 +1847,10 @@ int ssg_set_coupling(ssg_ctx* c, const ssg_ma_params* p) {
     }
     int rc = ma_check_params(c, p);
     if (rc) return rc;
+    if (p->stage_fp32 && c->precision != SSG_FP32_MIXED)
+        return set_err(c, SSG_ERR_ARG, "FP32 stage solves (stage_fp32) need an FP32-mixed context");
+    if (p->stage_fp32 && p->warm < 1)
+        return set_err(c, SSG_ERR_ARG, "FP32 stage solves (stage_fp32) start from the held history: need warm >= 1");
     HIP_OK(c, hipSetDevice(c->device));
     if ((rc = ma_alloc(c))) return rc;
     c->ma_par = *p;

@@ -61,6 +61,7 @@ EXPORTED_SYMBOLS = (
 
 MA_BODY = 0      # as wired: b on every level, surface slope 0
 MA_SURFACE = 1   # documented physics: surface slope = b
+MA_STAGE_TOL32 = 1e-6   # relative residual of the coupled step's FP32 stage solves (SSG_MA_STAGE_TOL32)
 
 
 class TimeScheme(enum.IntEnum):
@@ -105,7 +106,7 @@ class _Diag(ctypes.Structure):
 
 class _MaParams(ctypes.Structure):
     _fields_ = [("eps", ctypes.c_double), ("tol", ctypes.c_double), ("maxiter", ctypes.c_int32),
-                ("variant", ctypes.c_int32), ("warm", ctypes.c_int32)]
+                ("variant", ctypes.c_int32), ("warm", ctypes.c_int32), ("stage_fp32", ctypes.c_int32)]
 
 
 class _MaDiag(ctypes.Structure):
@@ -499,8 +500,8 @@ def diagnostics(domain: Domain) -> dict:
             "nonfinite": d.nonfinite, "ke_true": d.ke_true}
 
 
-def _ma_params(eps, tol, maxiter, variant, warm):
-    return _MaParams(float(eps), float(tol), int(maxiter), int(variant), int(warm))
+def _ma_params(eps, tol, maxiter, variant, warm, stage_fp32=False):
+    return _MaParams(float(eps), float(tol), int(maxiter), int(variant), int(warm), int(bool(stage_fp32)))
 
 
 def solve_monge_ampere(fields: Fields, domain: Domain, *, eps: float = 0.1, tol: float = 1e-10, maxiter: int = 50,
@@ -579,16 +580,19 @@ def get_phi3d(domain: Domain) -> np.ndarray:
 
 
 def set_coupling(domain: Domain, *, eps: float | None = 0.1, tol: float = 1e-10, maxiter: int = 50,
-                 variant: int = MA_BODY, warm: int = 2) -> None:
+                 variant: int = MA_BODY, warm: int = 2, stage_fp32: bool = False) -> None:
     """Coupled stepping: re-solve the Monge-Ampère problem at every RK stage
     (``eps=None`` restores the reference's frozen φₛ).  ``warm``: 0 cold
-    starts, 1 starts each solve from the last D(Φ), 2 from the linear
-    extrapolation of the last two (the stage abscissae are equally spaced)."""
+    starts, 1 starts each solve from the last D(Φ), 2 extrapolates over the
+    same stage of the last three steps.  ``stage_fp32`` (FP32-mixed domains):
+    the solves inside a ``timestep`` call run in FP32 to a relative residual
+    of ``MA_STAGE_TOL32``; the solve for the b the call returns stays FP64 to
+    ``tol``."""
     if eps is None:
         domain.check(domain.lib.ssg_set_coupling(domain.handle, None))
         domain.coupled = False
         return
-    p = _ma_params(eps, tol, maxiter, variant, warm)
+    p = _ma_params(eps, tol, maxiter, variant, warm, stage_fp32)
     domain.check(domain.lib.ssg_set_coupling(domain.handle, ctypes.byref(p)))
     domain.coupled = True
 
@@ -644,7 +648,7 @@ def load_state(domain: Domain, blob: bytes, state: "TimeState") -> None:
     # the blob restores the coupling mode; mirror it
     n2 = domain.Nx * domain.Ny
     ma = _STATE_HEADER_BYTES + n2 * (16 + (4 if domain.precision == SSG_FP32_MIXED else 8))
-    if blob[:8] == b"SSGSTAT2" and len(blob) >= ma + 28:
+    if blob[:8] in (b"SSGSTAT2", b"SSGSTAT3") and len(blob) >= ma + 28:
         domain.coupled = bool(int.from_bytes(blob[ma + 24:ma + 28], "little"))
 
 

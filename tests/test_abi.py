@@ -53,6 +53,7 @@ def test_struct_layouts_match_c():
     assert ssgamd._TimeParams.filter_strength.offset == 16
     assert ctypes.sizeof(ssgamd._TimeState) == 24
     assert ctypes.sizeof(ssgamd._MaDiag) == 552
+    assert ctypes.sizeof(ssgamd._MaParams) == 32 and ssgamd._MaParams.stage_fp32.offset == 28
     assert ctypes.sizeof(ssgamd._Diag) == 48 and ssgamd._Diag.ke_true.offset == 40
 
 

@@ -643,3 +643,81 @@ def test_vertical_velocity_against_oracle(gpu, nx, ny, nz, stretched):
     assert scale > 1e-6
     assert np.abs(w[0]).max() < 1e-14 * scale + 1e-300 and np.abs(w[-1]).max() < 1e-14 * scale + 1e-300
     assert np.abs(w - ref).max() < 1e-10 * scale, np.abs(w - ref).max() / scale
+
+
+def _coupled_stage_run(gpu, b0, nz, eps, dt, scheme, calls, prec, stage_fp32, tol=1e-10):
+    """Coupled steps in several ssg_timestep calls (each ends with the FP64
+    solve for the b it returns); returns b, φₛ and the surface kinetic energy
+    after each call."""
+    n = b0.shape[0]
+    d = gpu.make_domain(n, n, nz, precision=prec)
+    f = gpu.allocate_fields(d)
+    f.bs = b0
+    gpu.set_coupling(d, eps=eps, tol=tol, maxiter=60, variant=gpu.MA_BODY, warm=2, stage_fp32=stage_fp32)
+    st = gpu.TimeState(0.0, f)
+    p = gpu.TimeParams(dt, scheme=scheme, filter_freq=5, filter_strength=36.0, filter_order=8, filter_cutoff=0.65)
+    outs = []
+    for k in calls:
+        gpu.timestep(f, d, p, st, k)
+        outs.append((f.bs, f.phis, gpu.diagnostics(d)["ke_surface"]))
+    return d, f, outs
+
+
+@pytest.mark.parametrize("scheme", [1, 0])
+def test_coupled_stage_fp32_against_fp64(gpu, scheme, monkeypatch, capfd):
+    """FP32 stage solves (stage_fp32): the solves inside a timestep call run in
+    FP32 to SSG_MA_STAGE_TOL32, most on their first evaluation (read from the
+    solve trace); the call's last solve is FP64.  After 3 calls of
+    7 steps, b, φₛ and the surface kinetic energy stay within 1e-5 of the
+    FP64 coupled step, and φₛ and the kinetic energy after each call are
+    those of an FP64 solve of the returned b (1e-9 of a cold solve)."""
+    n, nz, eps, dt, calls = 128, 8, 0.1, 1e-3, (7, 7, 7)
+    b0 = smooth_b(n, 6, 6.0)
+    d64, _, ref = _coupled_stage_run(gpu, b0, nz, eps, dt, scheme, calls, gpu.SSG_FP64, False)
+    d64.close()
+    monkeypatch.setenv("SSG_MA_TRACE", "1")
+    capfd.readouterr()
+    d, f, got = _coupled_stage_run(gpu, b0, nz, eps, dt, scheme, calls, gpu.SSG_FP32_MIXED, True)
+    monkeypatch.delenv("SSG_MA_TRACE")
+    trace = [ln.split() for ln in capfd.readouterr().err.splitlines() if ln.startswith("ssg_ma_trace")]
+    fp32 = [t for t in trace if t[1] == "fp32"]
+    finals = [t for t in trace if t[1] == "fp64-final"]
+    solves_per_step = 3 if scheme == 1 else 1
+    # every stage solve after the first call's start is FP32; one FP64 solve ends each call
+    assert len(finals) == len(calls)
+    assert len(fp32) >= solves_per_step * sum(calls) - len(calls) - 2, trace
+    assert all(t[6] == "conv=1" for t in fp32)
+    evals = [int(t[2].split("=")[1]) for t in fp32]
+    # the first solves of a run extrapolate from a short history
+    assert sum(e == 1 for e in evals) >= 0.8 * len(evals) and max(evals) <= 4, evals
+    for (b32, p32, k32), (b64, p64, k64) in zip(got, ref):
+        eb = np.linalg.norm(b32 - b64) / np.linalg.norm(b64)
+        ep = np.linalg.norm(p32 - p64) / np.linalg.norm(p64)
+        ek = abs(k32 - k64) / abs(k64)
+        print(f"stage_fp32 vs fp64: b {eb:.2e} phis {ep:.2e} ke {ek:.2e}", flush=True)
+        assert eb < 1e-5 and ep < 1e-5 and ek < 1e-5
+        assert eb > 0
+    d2, f2, diag = gpu_solve(gpu, f.bs, nz, eps, gpu.MA_BODY, tol=1e-12)
+    assert diag["converged"]
+    assert np.abs(f.phis - f2.phis).max() < 1e-9 * np.abs(f2.phis).max()
+    k1, k2 = gpu.diagnostics(d)["ke_surface"], gpu.diagnostics(d2)["ke_surface"]
+    assert abs(k1 - k2) < 1e-9 * k2
+    d.close()
+    d2.close()
+
+
+def test_coupled_stage_fp32_rejects(gpu):
+    """stage_fp32 needs an FP32-mixed context and a warm start; other values
+    than 0 and 1 are rejected."""
+    d = gpu.make_domain(32, 32, 4)
+    with pytest.raises(gpu.SSGError):
+        gpu.set_coupling(d, eps=0.1, stage_fp32=True)
+    d.close()
+    d = gpu.make_domain(32, 32, 4, precision=gpu.SSG_FP32_MIXED)
+    with pytest.raises(gpu.SSGError):
+        gpu.set_coupling(d, eps=0.1, warm=0, stage_fp32=True)
+    p = gpu._ma_params(0.1, 1e-10, 50, gpu.MA_BODY, 2)
+    p.stage_fp32 = 2
+    assert d.lib.ssg_set_coupling(d.handle, ctypes.byref(p)) == gpu.SSG_ERR_ARG
+    gpu.set_coupling(d, eps=0.1, stage_fp32=True)
+    d.close()

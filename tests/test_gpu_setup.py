@@ -156,6 +156,38 @@ def test_coupled_restart_is_bitwise(gpu, prec, scheme, tmp_path):
         d2.close()
 
 
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_coupled_stage_fp32_restart_is_bitwise(gpu, scheme):
+    """With FP32 stage solves the restart blob carries their FP32 history
+    (widened to FP64 exactly): saved after 7 steps and loaded into a fresh
+    context, 6 more steps give bitwise the uninterrupted run's b, φₛ and 3D φ."""
+    from ssgamd.ics import spectral_b0
+    n, nz, eps = 128, 8, 0.1
+    b0 = spectral_b0(n, n, seed=5)
+    p = gpu.TimeParams(1e-3, scheme=scheme, filter_freq=3, filter_strength=36.0, filter_order=8, filter_cutoff=0.65)
+    d = gpu.make_domain(n, n, nz, precision=gpu.SSG_FP32_MIXED)
+    f = gpu.allocate_fields(d)
+    f.bs = b0
+    gpu.set_coupling(d, eps=eps, tol=1e-10, maxiter=60, variant=gpu.MA_BODY, warm=2, stage_fp32=True)
+    st = gpu.TimeState(0.0, f)
+    gpu.timestep(f, d, p, st, 7)
+    blob = gpu.save_state(d, st)
+    assert blob[:8] == b"SSGSTAT3"
+    gpu.timestep(f, d, p, st, 6)
+    ref_b, ref_phis, ref_phi = f.bs, f.phis, gpu.get_phi3d(d)
+    d.close()
+    d2 = gpu.make_domain(n, n, nz, precision=gpu.SSG_FP32_MIXED)
+    f2 = gpu.allocate_fields(d2)
+    st2 = gpu.TimeState(0.0, f2)
+    gpu.load_state(d2, blob, st2)
+    assert d2.coupled and st2.step == 7
+    gpu.timestep(f2, d2, p, st2, 6)
+    assert np.array_equal(f2.bs, ref_b)
+    assert np.array_equal(f2.phis, ref_phis)
+    assert np.array_equal(gpu.get_phi3d(d2), ref_phi)
+    d2.close()
+
+
 def test_coupled_adjoint_from_restart(gpu):
     """The coupled adjoint started after loading a checkpoint equals the one
     started in the run that wrote it (same start state, bitwise)."""
