@@ -279,6 +279,34 @@ __device__ __forceinline__ void st_plane(cpx<R>* p, cpx<R> x, bool nt) {
     else
         *p = x;
 }
+// E elements p[T*m], m = 0..E-1, in one straight run of loads (stores): the
+// access-mode branch sits outside the unrolled loop, so no load waits for
+// the one before it.
+template <typename R, int E>
+__device__ __forceinline__ void ld_plane_n(cpx<R> (&v)[E], const cpx<R>* p, int T, bool nt) {
+    using V = typename vec2_t<R>::type;
+    if (nt) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const V w = __builtin_nontemporal_load(reinterpret_cast<const V*>(p + T * m));
+            v[m] = cpx<R>{w.x, w.y};
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = p[T * m];
+    }
+}
+template <typename R, int E>
+__device__ __forceinline__ void st_plane_n(cpx<R>* p, const cpx<R> (&v)[E], int T, bool nt) {
+    using V = typename vec2_t<R>::type;
+    if (nt) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) __builtin_nontemporal_store(V{v[m].x, v[m].y}, reinterpret_cast<V*>(p + T * m));
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) p[T * m] = v[m];
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Column pass: per kx column, y-FFT -> symbol -> inverse y-FFT.
@@ -367,12 +395,24 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
     C (&vb)[E] = vv[1];
     const C* srcA = a.inA + (size_t)(live ? line : 0) * (a.in_stride ? (size_t)a.in_stride : (size_t)pitch);
     constexpr bool MA_PLANES = (OP == OP_MA_DERIV || OP == OP_MA_DIFF);
+    // OP_MA_DIFF: D̂_prev of this line, loaded with the input (it does not
+    // depend on the transform)
+    C prev[OP == OP_MA_DIFF ? E : 1];
+    cpx<double> bfor[OP == OP_MA_DIFF ? E : 1];  // and the forcing of its kx column (body variant)
+    if constexpr (MA_PLANES) {
+        // lines past the end read line 0 (in bounds) and store nothing
+        ld_plane_n<R, E>(va, srcA + t, T, a.nt);
+        if constexpr (OP == OP_MA_DIFF) {
+            ld_plane_n<R, E>(prev, a.outA + (size_t)(live ? line : 0) * pitch + t, T, a.nt);
+            if (a.body) {
+                const cpx<double>* b2p = a.B2 + (size_t)(live ? c : 0) * a.g.ny + t;
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-        if constexpr (MA_PLANES)
-            va[m] = live ? ld_plane<R>(srcA + t + T * m, a.nt) : C{0, 0};
-        else
-            va[m] = live ? srcA[t + T * m] : C{0, 0};
+                for (int m = 0; m < E; ++m) bfor[m] = b2p[T * m];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) va[m] = live ? srcA[t + T * m] : C{0, 0};
     }
     if constexpr (TWO_IN) {
         const C* srcB = a.inB + (size_t)(live ? line : 0) * pitch;
@@ -382,12 +422,11 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
     auto store = [&](C* dst, const C (&v)[E]) {
         if (live) {
             C* d = dst + (size_t)line * pitch;
+            if constexpr (MA_PLANES) {
+                st_plane_n<R, E>(d + t, v, T, a.nt);
+            } else {
 #pragma unroll
-            for (int m = 0; m < E; ++m) {
-                if constexpr (MA_PLANES)
-                    st_plane<R>(d + t + T * m, v[m], a.nt);
-                else
-                    d[t + T * m] = v[m];
+                for (int m = 0; m < E; ++m) d[t + T * m] = v[m];
             }
         }
     };
@@ -508,28 +547,21 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
             const bool interior = live && lev >= 1 && lev <= a.nlev - 2;
             const double wgt = (c == 0 || 2 * c == a.g.nx) ? 1.0 : 2.0;
             double acc[2] = {0.0, 0.0};
-            C* dp = a.outA + (size_t)(live ? line : 0) * pitch;
-            const cpx<double>* b2p = a.B2 + (size_t)c * a.g.ny;
+            if (interior) {
 #pragma unroll
-            for (int m = 0; m < E; ++m) {
-                const int y = t + T * m;
-                if (live) {
-                    const C prev = ld_plane<R>(dp + y, a.nt);
-                    if (interior) {
-                        const double rx = a.eps * ((double)prev.x - (double)va[m].x);
-                        const double ry = a.eps * ((double)prev.y - (double)va[m].y);
-                        double hx = a.eps * (double)va[m].x, hy = a.eps * (double)va[m].y;
-                        if (a.body) {
-                            const cpx<double> bb = b2p[y];
-                            hx += bb.x;
-                            hy += bb.y;
-                        }
-                        acc[0] += wgt * (rx * rx + ry * ry);
-                        acc[1] += wgt * (hx * hx + hy * hy);
+                for (int m = 0; m < E; ++m) {
+                    const double rx = a.eps * ((double)prev[m].x - (double)va[m].x);
+                    const double ry = a.eps * ((double)prev[m].y - (double)va[m].y);
+                    double hx = a.eps * (double)va[m].x, hy = a.eps * (double)va[m].y;
+                    if (a.body) {
+                        hx += bfor[m].x;
+                        hy += bfor[m].y;
                     }
-                    st_plane<R>(dp + y, va[m], a.nt);
+                    acc[0] += wgt * (rx * rx + ry * ry);
+                    acc[1] += wgt * (hx * hx + hy * hy);
                 }
             }
+            store(a.outA, va);
             block_sum_g<2>(acc, a.partial, bidx);
             return;
         } else if constexpr (OP == OP_ADJ) {

@@ -225,89 +225,156 @@ struct MaTriArgs {
     Grid g;
 };
 
-template <typename R, typename P, int NZMAX>
-__global__ __launch_bounds__(128) void k_ma_tridiag(MaTriArgs<R, P> a) {
+// Two lanes per wavenumber (a twisted factorisation): the even lane
+// eliminates upward from the bottom row through levels 0..m-1, the odd lane
+// downward from the top row through levels nz-1..m (m = nz/2), both with the
+// same code on mirrored coefficients; the two interface rows are solved
+// after exchanging their reduced forms (lane shuffles), and each lane
+// back-substitutes its own half.  Half the registers and half the dependent
+// chain of one Thomas sweep per wavenumber.  NZH >= ceil(nz/2).
+template <typename R, typename P, int NZH>
+__global__ __launch_bounds__(256) void k_ma_tridiag(MaTriArgs<R, P> a) {
+    // the stencil table (3 nz + 6 values, nz <= 2 NZH) in LDS: its reads
+    // cost no vector-memory waits in the sweep
+    __shared__ double cf[3 * 2 * NZH + 6];
+    for (int i = threadIdx.x; i < 3 * a.nz + 6; i += blockDim.x) cf[i] = a.coef[i];
+    __syncthreads();
     const long total = (long)a.ncols * a.g.ny;
-    const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (q >= total) return;
-    const int ny = a.g.ny, nz = a.nz;
+    const long q = (blockIdx.x * (long)blockDim.x + threadIdx.x) >> 1;
+    if (q >= total) return;  // both lanes of a pair leave together
+    const bool up = (threadIdx.x & 1) != 0;
+    const int ny = a.g.ny, nz = a.nz, m = nz / 2;
+    const int cnt = up ? nz - m : m;
     const int kx = (int)(q / ny), ky = (int)(q - (long)kx * ny);
     const int s = signed_k(ky, ny);
     const double hs = (double)sym_dxx<double>(a.g, kx) + (double)sym_dyy<double>(a.g, s);
-    const double* cf = a.coef;
     const cpx<double> b2 = a.B2[(size_t)kx * ny + ky];
     const size_t base = (size_t)kx * nz * ny + ky;
-    auto rhs_at = [&](int k) -> double2 {
+    // level of sweep position j: 0, 1, ... from the bottom (even lane) or
+    // nz-1, nz-2, ... from the top (odd lane); j >= 1 are interior rows
+    auto lev = [&](int j) { return up ? nz - 1 - j : j; };
+    double cc[NZH];
+    double2 dd[NZH];
+    // every D̂ value the lane needs in one straight run of loads: its interior
+    // rows (dd[j] holds the right-hand side until the sweep overwrites it)
+    // and, in slot 0, the interior row next to its boundary row.  Positions
+    // past this lane's rows load a clamped, in-bounds level and are unused.
+    cpx<R> dv[NZH];
+    if (a.use_d) {
+        const cpx<R>* col = a.Dh + base;
+        int lv[NZH];
+#pragma unroll
+        for (int j = 0; j < NZH; ++j) {
+            const int k = j == 0 ? (up ? nz - 2 : 1) : lev(j);
+            lv[j] = k < 0 ? 0 : (k > nz - 1 ? nz - 1 : k);
+        }
+        using V = typename vec2_t<R>::type;
+        if (a.nt) {
+#pragma unroll
+            for (int j = 0; j < NZH; ++j) {
+                const V w = __builtin_nontemporal_load(reinterpret_cast<const V*>(col + (size_t)lv[j] * ny));
+                dv[j] = cpx<R>{w.x, w.y};
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < NZH; ++j) dv[j] = col[(size_t)lv[j] * ny];
+        }
+    }
+    auto rhs_of = [&](int j) -> double2 {
         double2 f = {0.0, 0.0};
         if (a.body) f = b2;
         if (a.use_d) {
-            const cpx<R> d = ld_plane<R>(a.Dh + base + (size_t)k * ny, a.nt);
-            f.x += a.eps * (double)d.x;
-            f.y += a.eps * (double)d.y;
+            f.x += a.eps * (double)dv[j].x;
+            f.y += a.eps * (double)dv[j].y;
         }
         return f;
     };
-    double cp[NZMAX];
-    double2 dp[NZMAX];
-    // Every right-hand side f_k (interior levels 1..nz-2) is loaded into
-    // dp[k] before the sweep: the loads are independent of the recurrence,
-    // so all of them are in flight at once instead of one per level.  The
-    // sweep then overwrites dp[k] in place (f_k is read before dp[k] is
-    // written); the top row still needs f_{nz-2}, kept in f_top.
-    double2 f_top = {0.0, 0.0};
 #pragma unroll
-    for (int k = 1; k < NZMAX; ++k) {
-        if (k < nz - 1) {
-            dp[k] = rhs_at(k);
-            if (k == nz - 2) f_top = dp[k];
-        }
-    }
-    // row 0: bottom Neumann row with Φ2 eliminated through interior row 1
+    for (int j = 1; j < NZH; ++j) dd[j] = rhs_of(j);
+    // the boundary row of this lane, with the next level eliminated through
+    // the adjacent interior row (src/poisson.jl:309-364)
+    const double2 fb = rhs_of(0);
     {
-        const double A0 = cf[3 * nz], B0 = cf[3 * nz + 1], C0 = cf[3 * nz + 2];
-        const double al = cf[3], be = cf[4] + hs, ga = cf[5];
-        const double2 f1 = dp[1];
-        double a0 = A0 - C0 * al / ga, b0 = B0 - C0 * be / ga;
-        double2 r0 = {-C0 * f1.x / ga, -C0 * f1.y / ga};
-        if (kx == 0 && ky == 0) {  // gauge of the Neumann-Neumann mean mode
-            a0 = 1.0;
-            b0 = 0.0;
-            r0 = {0.0, 0.0};
-        }
-        cp[0] = b0 / a0;
-        dp[0] = {r0.x / a0, r0.y / a0};
-    }
-    // interior rows, then the top Neumann row with Φn-3 eliminated through
-    // interior row n-2 (indices stay compile-time so cp/dp live in registers)
-#pragma unroll
-    for (int k = 1; k < NZMAX; ++k) {
-        if (k < nz - 1) {
-            const double al = cf[3 * k], be = cf[3 * k + 1] + hs, ga = cf[3 * k + 2];
-            const double2 f = dp[k];
-            const double rm = 1.0 / (be - al * cp[k - 1]);
-            cp[k] = ga * rm;
-            dp[k] = {(f.x - al * dp[k - 1].x) * rm, (f.y - al * dp[k - 1].y) * rm};
-        } else if (k == nz - 1) {
-            const int ki = k - 1;
+        double diag, next;
+        double2 f;
+        if (!up) {  // bottom: A0 Φ0 + B0 Φ1 + C0 Φ2 = 0
+            const double A0 = cf[3 * nz], B0 = cf[3 * nz + 1], C0 = cf[3 * nz + 2];
+            const double al = cf[3], be = cf[4] + hs, ga = cf[5];
+            diag = A0 - C0 * al / ga;
+            next = B0 - C0 * be / ga;
+            f = {-C0 * fb.x / ga, -C0 * fb.y / ga};
+            if (kx == 0 && ky == 0) {  // gauge of the Neumann-Neumann mean mode
+                diag = 1.0;
+                next = 0.0;
+                f = {0.0, 0.0};
+            }
+        } else {  // top: a Φn-1 + b Φn-2 + c Φn-3 = g
+            const int ki = nz - 2;
             const double at = cf[3 * nz + 3], bt = cf[3 * nz + 4], ct = cf[3 * nz + 5];
             const double al = cf[3 * ki], be = cf[3 * ki + 1] + hs, ga = cf[3 * ki + 2];
-            const double2 f = f_top;
+            diag = at - ct * ga / al;
+            next = bt - ct * be / al;
             double2 g = {0.0, 0.0};
             if (a.top) g = b2;
-            const double lo = bt - ct * be / al, di = at - ct * ga / al;
-            const double2 rr = {g.x - ct * f.x / al, g.y - ct * f.y / al};
-            const double rm = 1.0 / (di - lo * cp[k - 1]);
-            cp[k] = 0.0;
-            dp[k] = {(rr.x - lo * dp[k - 1].x) * rm, (rr.y - lo * dp[k - 1].y) * rm};
+            f = {g.x - ct * fb.x / al, g.y - ct * fb.y / al};
+        }
+        cc[0] = next / diag;
+        dd[0] = {f.x / diag, f.y / diag};
+    }
+    // interior rows in sweep order: `pv` couples to the level already
+    // eliminated, `nx` to the next one
+    // Every position is swept, also those past this lane's rows (j >= cnt,
+    // levels clamped into the stencil table, results unused): without a
+    // per-row predicate the stencil loads can all be issued ahead of the
+    // dependent chain.
+#pragma unroll
+    for (int j = 1; j < NZH; ++j) {
+        const int k0 = lev(j), k = k0 < 1 ? 1 : (k0 > nz - 2 ? nz - 2 : k0);
+        const double al = cf[3 * k], be = cf[3 * k + 1] + hs, ga = cf[3 * k + 2];
+        const double pv = up ? ga : al, nx = up ? al : ga;
+        const double2 f = dd[j];
+        const double rm = 1.0 / (be - pv * cc[j - 1]);
+        cc[j] = nx * rm;
+        dd[j] = {(f.x - pv * dd[j - 1].x) * rm, (f.y - pv * dd[j - 1].y) * rm};
+    }
+    // interface rows: Φ(m-1) + cp Φ(m) = dp (even lane), Φ(m) + cq Φ(m-1) = dq
+    // (odd lane); both lanes solve the pair with the same operations
+    double cl = 0.0;
+    double2 dl = {0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < NZH; ++j)
+        if (j == cnt - 1) {
+            cl = cc[j];
+            dl = dd[j];
+        }
+    const double co = __shfl_xor(cl, 1);
+    const double2 dox = {__shfl_xor(dl.x, 1), __shfl_xor(dl.y, 1)};
+    const double cp = up ? co : cl, cq = up ? cl : co;
+    const double2 dp = up ? dox : dl, dq = up ? dl : dox;
+    const double den = 1.0 / (1.0 - cq * cp);
+    const double2 xm = {(dq.x - cq * dp.x) * den, (dq.y - cq * dp.y) * den};
+    const double2 xm1 = {dp.x - cp * xm.x, dp.y - cp * xm.y};
+    // back-substitution of this lane's half, from the interface outward
+    double2 xn = up ? xm1 : xm;
+    cpx<P> xo[NZH];
+#pragma unroll
+    for (int j = NZH - 1; j >= 0; --j) {
+        if (j < cnt) {  // (the positions past this lane's rows hold no solution)
+            const double2 x = {dd[j].x - cc[j] * xn.x, dd[j].y - cc[j] * xn.y};
+            xo[j] = cpx<P>{(P)x.x, (P)x.y};
+            xn = x;
         }
     }
-    double2 x = {0.0, 0.0};
+    cpx<P>* out = a.Phi + base;
+    if (a.nt) {
+        using V = typename vec2_t<P>::type;
 #pragma unroll
-    for (int k = NZMAX - 1; k >= 0; --k) {
-        if (k <= nz - 1) {
-            x = {dp[k].x - cp[k] * x.x, dp[k].y - cp[k] * x.y};
-            st_plane<P>(a.Phi + base + (size_t)k * ny, cpx<P>{(P)x.x, (P)x.y}, a.nt);
-        }
+        for (int j = 0; j < NZH; ++j)
+            if (j < cnt) __builtin_nontemporal_store(V{xo[j].x, xo[j].y}, reinterpret_cast<V*>(out + (size_t)lev(j) * ny));
+    } else {
+#pragma unroll
+        for (int j = 0; j < NZH; ++j)
+            if (j < cnt) out[(size_t)lev(j) * ny] = xo[j];
     }
 }
 

@@ -247,20 +247,21 @@ int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d, const cpx<R>* Dh, 
     a.nt = ma_stream_planes(c);
     a.g = c->g;
     const long total = (long)c->ncols_full * c->ny;
-    const int nblk = (int)((total + 127) / 128);
+    // two lanes per wavenumber
+    const int nblk = (int)((2 * total + 255) / 256);
     const double bytes = (double)total * (c->nz * (sizeof(cpx<R>) + sizeof(cpx<P>)) + sizeof(cpx<double>));
     int slot, rc;
     if ((rc = prof_begin(c, &slot))) return rc;
     if (c->nz <= 8)
-        launch_on(c, slot, k_ma_tridiag<R, P, 8>, dim3(nblk), dim3(128), 0, a);
+        launch_on(c, slot, k_ma_tridiag<R, P, 4>, dim3(nblk), dim3(256), 0, a);
     else if (c->nz <= 16)
-        launch_on(c, slot, k_ma_tridiag<R, P, 16>, dim3(nblk), dim3(128), 0, a);
+        launch_on(c, slot, k_ma_tridiag<R, P, 8>, dim3(nblk), dim3(256), 0, a);
     else if (c->nz <= 32)
-        launch_on(c, slot, k_ma_tridiag<R, P, 32>, dim3(nblk), dim3(128), 0, a);
+        launch_on(c, slot, k_ma_tridiag<R, P, 16>, dim3(nblk), dim3(256), 0, a);
     else if (c->nz <= 64)
-        launch_on(c, slot, k_ma_tridiag<R, P, 64>, dim3(nblk), dim3(128), 0, a);
+        launch_on(c, slot, k_ma_tridiag<R, P, 32>, dim3(nblk), dim3(256), 0, a);
     else
-        launch_on(c, slot, k_ma_tridiag<R, P, 128>, dim3(nblk), dim3(128), 0, a);
+        launch_on(c, slot, k_ma_tridiag<R, P, 64>, dim3(nblk), dim3(256), 0, a);
     c->launches++;
     HIP_OK(c, hipGetLastError());
     return prof_end(c, slot, std::is_same<R, float>::value ? 12 : 4, bytes);
