@@ -131,6 +131,7 @@ struct ssg_ctx {
     SolveHistory32 ma_hist32;
     bool ma_have_d32 = false;
     int ma_ept[2] = {0, 0};           // elements per thread of k_ma_rows per path (0 FP64, 1 FP32)
+    int ma_cols_ept[2] = {0, 0};      // ... of the Monge-Ampère column passes (0: the tendency's choice)
     Grid g3{};                        // 3D plane geometry: pitch = nz * ny
     std::vector<double> dz;           // vertical spacings (Nz entries) of a custom grid; empty: uniform Lz/(Nz-1)
     // vertical velocity (allocated on first use): Φxx, Φyy, Φxy per level,
@@ -337,9 +338,9 @@ int launch_rows_fwd(ssg_ctx* c, const S* f, const cpx<R>* w, cpx<R>* PA, cpx<R>*
 // General column-pass launcher: `a` carries the operands; lines are
 // (kx, level) pairs when nlev > 1 (kx = line / nlev).
 template <typename R, int OP>
-int launch_cols_args(ssg_ctx* c, ColsArgs<R> a, int nlines, int nlev) {
+int launch_cols_args(ssg_ctx* c, ColsArgs<R> a, int nlines, int nlev, int ept_force = 0) {
     PathBuffers<R>& B = bufs<R>(c);
-    const int ept = ept_of<R>(c, 1, c->ny);
+    const int ept = ept_force ? ept_force : ept_of<R>(c, 1, c->ny);
     return with_size_ept(c->ny, ept, [&](auto tag, auto etag) -> int {
         constexpr int M = decltype(tag)::value;
         constexpr int E = decltype(etag)::value;

@@ -279,6 +279,15 @@ int ma_ept_of(ssg_ctx* c) {
     return ept_valid(c->nx, 8) ? 8 : ept_default(c->nx);
 }
 
+// elements per thread of the Monge-Ampère column passes (OP_MA_DERIV,
+// OP_MA_DIFF): their own tuned choice, else the tendency's column-pass one
+template <typename R>
+int ma_cols_ept_of(ssg_ctx* c) {
+    const int p = std::is_same<R, float>::value ? 1 : 0;
+    const int e = c->ma_cols_ept[p];
+    return (e && ept_valid(c->ny, e)) ? e : ept_of<R>(c, 1, c->ny);
+}
+
 template <typename R>
 int ma_rows(ssg_ctx* c, cpx<R>* const (&planes)[3]) {
     PathBuffers<R>& B = bufs<R>(c);
@@ -312,6 +321,86 @@ int cpx_convert(ssg_ctx* c, const cpx<A>* in, cpx<B>* out, size_t n) {
 }
 
 std::map<std::tuple<int, int, int, int, int>, int> g_ma_tune_cache;
+std::map<std::tuple<int, int, int, int, int>, int> g_ma_cols_tune_cache;
+
+// The Monge-Ampère column passes' elements per thread for path R: one
+// derivative pass and one residual pass per candidate (4, 8, 16), timed
+// twice after an untimed round, on scratch planes (the residual pass writes
+// planes[1], not the solver's D̂).  SSG_MA_COLS_EPT=e forces a choice.
+template <typename R>
+int ma_tune_cols(ssg_ctx* c, const cpx<R>* phi) {
+    const int p = std::is_same<R, float>::value ? 1 : 0;
+    if (c->ma_cols_ept[p]) return SSG_OK;
+    if (const char* env = getenv("SSG_MA_COLS_EPT")) {
+        c->ma_cols_ept[p] = atoi(env);
+        return SSG_OK;
+    }
+    const char* tv = getenv("SSG_TUNE");
+    if ((tv && tv[0] == '0') || (long)c->nx * c->ny * c->nz < 256L * 256L * 8L) return SSG_OK;
+    const auto key = std::make_tuple(c->nx, c->ny, c->nz, p, c->device);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_ma_cols_tune_cache.find(key);
+        if (it != g_ma_cols_tune_cache.end()) {
+            c->ma_cols_ept[p] = it->second;
+            return SSG_OK;
+        }
+    }
+    const bool prof_saved = c->prof;
+    c->prof = false;
+    cpx<R>* const pl[3] = {(cpx<R>*)c->ma_p[0], (cpx<R>*)c->ma_p[1], (cpx<R>*)c->ma_p[2]};
+    const int lines = c->ncols_full * c->nz;
+    ColsArgs<R> da{};
+    da.inA = phi;
+    da.outA = pl[0];
+    da.outB = pl[1];
+    da.outC = pl[2];
+    da.scale = (R)1;
+    da.nt = ma_stream_planes(c);
+    ColsArgs<R> ra{};
+    ra.inA = pl[0];
+    ra.outA = pl[1];
+    ra.B2 = c->ma_b2;
+    ra.eps = 0.1;
+    ra.body = 1;
+    ra.partial = c->ma_partial;
+    ra.nt = da.nt;
+    const int cands[3] = {4, 8, 16};
+    int best = 0, rc = SSG_OK;
+    double best_ms = 1e30;
+    for (int ci = 0; ci < 3 && rc == SSG_OK; ++ci) {
+        const int e = cands[ci];
+        if (!ept_valid(c->ny, e)) continue;
+        auto pass = [&]() -> int {
+            int r = launch_cols_args<R, OP_MA_DERIV>(c, da, lines, c->nz, e);
+            return r ? r : launch_cols_args<R, OP_MA_DIFF>(c, ra, lines, c->nz, e);
+        };
+        if ((rc = pass())) break;
+        if (hipEventRecord(c->ev0, c->stream) != hipSuccess) {
+            rc = set_err(c, SSG_ERR_HIP, "tuning timing failed");
+            break;
+        }
+        for (int i = 0; i < 2 && rc == SSG_OK; ++i) rc = pass();
+        if (rc) break;
+        float ms = 0;
+        if (hipEventRecord(c->ev1, c->stream) != hipSuccess || hipEventSynchronize(c->ev1) != hipSuccess ||
+            hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) {
+            rc = set_err(c, SSG_ERR_HIP, "tuning timing failed");
+            break;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = e;
+        }
+    }
+    c->prof = prof_saved;
+    c->ma_cols_ept[p] = best;
+    if (rc == SSG_OK && best) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_ma_cols_tune_cache.emplace(key, best);
+    }
+    return rc;
+}
 
 // Pick k_ma_rows' elements per thread for path R on this grid: two timed
 // launches per candidate after one untimed one, on the (allocated) planes.
@@ -428,7 +517,7 @@ int ma_eval(ssg_ctx* c, const ssg_ma_params* p, const cpx<R>* phi, cpx<R>* const
     a.scale = (R)scale;
     a.nt = ma_stream_planes(c);
     int rc;
-    if ((rc = launch_cols_args<R, OP_MA_DERIV>(c, a, lines, c->nz))) return rc;
+    if ((rc = launch_cols_args<R, OP_MA_DERIV>(c, a, lines, c->nz, ma_cols_ept_of<R>(c)))) return rc;
     if ((rc = ma_rows<R>(c, planes))) return rc;
     ColsArgs<R> b{};
     b.inA = planes[0];
@@ -438,7 +527,7 @@ int ma_eval(ssg_ctx* c, const ssg_ma_params* p, const cpx<R>* phi, cpx<R>* const
     b.body = p->variant == SSG_MA_BODY ? 1 : 0;
     b.partial = c->ma_partial;
     b.nt = a.nt;
-    if ((rc = launch_cols_args<R, OP_MA_DIFF>(c, b, lines, c->nz))) return rc;
+    if ((rc = launch_cols_args<R, OP_MA_DIFF>(c, b, lines, c->nz, ma_cols_ept_of<R>(c)))) return rc;
     double r2 = 0, h2 = 0;
     if ((rc = ma_read_sums(c, &r2, &h2))) return rc;
     // Parseval: sum of r² over the grid = (1/(nx ny)) sum of |r̂|² per level
@@ -466,6 +555,8 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
     const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
     int rc0 = ma_tune<double>(c);
     if (rc0 == SSG_OK && c->precision == SSG_FP32_MIXED) rc0 = ma_tune<float>(c);
+    if (rc0 == SSG_OK) rc0 = ma_tune_cols<double>(c, c->ma_phi);
+    if (rc0 == SSG_OK && c->precision == SSG_FP32_MIXED) rc0 = ma_tune_cols<float>(c, c->ma32_phi);
     if (rc0) return rc0;
     const bool warm = p->warm && c->ma_have_d;
     const bool extrap = warm && p->warm >= 2;
