@@ -279,6 +279,16 @@ __device__ __forceinline__ void st_plane(cpx<R>* p, cpx<R> x, bool nt) {
     else
         *p = x;
 }
+// Row-tiled Monge-Ampère planes (the derivative planes and D between the
+// column and row passes): element (kx, row r = level*ny + y) of a plane of
+// kxn columns at ((r >> s) * kxn + kx) << s | (r & (2^s - 1)), s = log2 of the
+// tile's rows.  A row-pass block owns one tile, [kx][row in tile], a single
+// contiguous run; a column-pass block of consecutive kx writes runs of
+// (lines x tile rows) elements.
+__device__ __forceinline__ long ma_tile_off(long r, int kx, int kxn, int s) {
+    return (((r >> s) * kxn + kx) << s) | (r & ((1L << s) - 1));
+}
+
 // E elements p[T*m], m = 0..E-1, in one straight run of loads (stores): the
 // access-mode branch sits outside the unrolled loop, so no load waits for
 // the one before it.
@@ -294,6 +304,32 @@ __device__ __forceinline__ void ld_plane_n(cpx<R> (&v)[E], const cpx<R>* p, int 
     } else {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = p[T * m];
+    }
+}
+// the same at element offsets off[m]
+template <typename R, int E>
+__device__ __forceinline__ void ld_plane_at(cpx<R> (&v)[E], const cpx<R>* p, const long (&off)[E], bool nt) {
+    using V = typename vec2_t<R>::type;
+    if (nt) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const V w = __builtin_nontemporal_load(reinterpret_cast<const V*>(p + off[m]));
+            v[m] = cpx<R>{w.x, w.y};
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = p[off[m]];
+    }
+}
+template <typename R, int E>
+__device__ __forceinline__ void st_plane_at(cpx<R>* p, const cpx<R> (&v)[E], const long (&off)[E], bool nt) {
+    using V = typename vec2_t<R>::type;
+    if (nt) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) __builtin_nontemporal_store(V{v[m].x, v[m].y}, reinterpret_cast<V*>(p + off[m]));
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) p[off[m]] = v[m];
     }
 }
 template <typename R, int E>
@@ -328,6 +364,7 @@ struct ColsArgs {
     Grid g;
     long in_stride;         // elements between the lines of inA (0: ny, contiguous lines)
     int nt;                 // OP_MA_DERIV / OP_MA_DIFF: nontemporal plane access (ld_plane)
+    int yt_log2;            // OP_MA_DERIV out / OP_MA_DIFF in: log2 of the row tile of those planes
     // Monge-Ampere extensions (lines are (kx, level) pairs, kx = line / nlev)
     int nlev;
     cpx<R>* outC;
@@ -384,7 +421,22 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
     const int line = bidx * a.G + l;
     const bool live = line < a.ncols;
     const int pitch = a.g.ny;                         // lines are contiguous runs of ny
-    const int c = a.nlev > 1 ? line / a.nlev : line;  // kx column of this line
+    constexpr bool MA_PLANES = (OP == OP_MA_DERIV || OP == OP_MA_DIFF);
+    // Monge-Ampère passes: lines run level-major (line = level * kxn + kx), so
+    // the lines of a block are consecutive kx of one level; lines past the
+    // end stand in for line 0 (in bounds, nothing stored)
+    const int kxn = (a.g.nx >> 1) + 1;
+    const int mlev = MA_PLANES ? (live ? line : 0) / kxn : 0;
+    const int c = MA_PLANES ? (live ? line : 0) - mlev * kxn
+                            : (a.nlev > 1 ? line / a.nlev : line);  // kx column of this line
+    // offset of this (kx, level) line in the [kx][level][y] planes, and of its
+    // elements in the row-tiled planes
+    const size_t mline = ((size_t)c * a.nlev + mlev) * pitch;
+    long toff[MA_PLANES ? E : 1];
+    if constexpr (MA_PLANES) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) toff[m] = ma_tile_off((long)mlev * pitch + t + T * m, c, kxn, a.yt_log2);
+    }
     C* const buf1[1] = {lds + (l * NBUF) * LE};
     typename F::Twiddles tw;
     tw.load(t, a.twy);
@@ -394,21 +446,19 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
     C (&va)[E] = vv[0];
     C (&vb)[E] = vv[1];
     const C* srcA = a.inA + (size_t)(live ? line : 0) * (a.in_stride ? (size_t)a.in_stride : (size_t)pitch);
-    constexpr bool MA_PLANES = (OP == OP_MA_DERIV || OP == OP_MA_DIFF);
     // OP_MA_DIFF: D̂_prev of this line, loaded with the input (it does not
     // depend on the transform)
     C prev[OP == OP_MA_DIFF ? E : 1];
     cpx<double> bfor[OP == OP_MA_DIFF ? E : 1];  // and the forcing of its kx column (body variant)
-    if constexpr (MA_PLANES) {
-        // lines past the end read line 0 (in bounds) and store nothing
-        ld_plane_n<R, E>(va, srcA + t, T, a.nt);
-        if constexpr (OP == OP_MA_DIFF) {
-            ld_plane_n<R, E>(prev, a.outA + (size_t)(live ? line : 0) * pitch + t, T, a.nt);
-            if (a.body) {
-                const cpx<double>* b2p = a.B2 + (size_t)(live ? c : 0) * a.g.ny + t;
+    if constexpr (OP == OP_MA_DERIV) {
+        ld_plane_n<R, E>(va, a.inA + mline + t, T, a.nt);
+    } else if constexpr (OP == OP_MA_DIFF) {
+        ld_plane_at<R, E>(va, a.inA, toff, a.nt);
+        ld_plane_n<R, E>(prev, a.outA + mline + t, T, a.nt);
+        if (a.body) {
+            const cpx<double>* b2p = a.B2 + (size_t)c * a.g.ny + t;
 #pragma unroll
-                for (int m = 0; m < E; ++m) bfor[m] = b2p[T * m];
-            }
+            for (int m = 0; m < E; ++m) bfor[m] = b2p[T * m];
         }
     } else {
 #pragma unroll
@@ -421,10 +471,12 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
     }
     auto store = [&](C* dst, const C (&v)[E]) {
         if (live) {
-            C* d = dst + (size_t)line * pitch;
-            if constexpr (MA_PLANES) {
-                st_plane_n<R, E>(d + t, v, T, a.nt);
+            if constexpr (OP == OP_MA_DERIV) {
+                st_plane_at<R, E>(dst, v, toff, a.nt);
+            } else if constexpr (OP == OP_MA_DIFF) {
+                st_plane_n<R, E>(dst + mline + t, v, T, a.nt);
             } else {
+                C* d = dst + (size_t)line * pitch;
 #pragma unroll
                 for (int m = 0; m < E; ++m) d[t + T * m] = v[m];
             }
@@ -543,8 +595,7 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
         } else if constexpr (OP == OP_MA_DIFF) {
             // va = D-hat of this (kx, level) line; the residual of the iterate
             // that produced it is eps * (D_prev - D) on interior levels
-            const int lev = a.nlev > 1 ? line - c * a.nlev : 0;
-            const bool interior = live && lev >= 1 && lev <= a.nlev - 2;
+            const bool interior = live && mlev >= 1 && mlev <= a.nlev - 2;
             const double wgt = (c == 0 || 2 * c == a.g.nx) ? 1.0 : 2.0;
             double acc[2] = {0.0, 0.0};
             if (interior) {

@@ -26,6 +26,7 @@ struct MaRowsArgs {
     int nblk;
     const cpx<R>* twx;
     Grid g;      // pitch = nz * ny
+    int yt_log2; // row tile of the three input planes and of Dx (ma_tile_off)
 };
 
 // gather the packed spectrum of rows (A, B) of one plane into the line buffers
@@ -40,6 +41,22 @@ __device__ __forceinline__ void ma_load(const cpx<R>* P, int pitch, int y0, int 
             const size_t o = (size_t)kx * pitch + y0 + ll;
             q0[it] = P[o];
             q1[it] = P[o + G];
+        }
+    }
+}
+
+// the same from a row-tiled plane (ma_tile_off, tiles of 2^s rows of kxn
+// columns): rows A = y0 + l and B = y0 + G + l
+template <typename R, int N, int E, int MAXIT>
+__device__ __forceinline__ void ma_load_tile(const cpx<R>* P, int kxn, int s, int y0, int G, int total,
+                                             cpx<R> (&q0)[MAXIT], cpx<R> (&q1)[MAXIT]) {
+#pragma unroll
+    for (int it = 0; it < MAXIT; ++it) {
+        const int idx = threadIdx.x + it * (int)blockDim.x;
+        if (idx < total) {
+            const int kx = idx / G, ll = idx - kx * G;
+            q0[it] = P[ma_tile_off(y0 + ll, kx, kxn, s)];
+            q1[it] = P[ma_tile_off(y0 + G + ll, kx, kxn, s)];
         }
     }
 }
@@ -93,10 +110,13 @@ __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
     typename F::Twiddles tw;
     tw.load(t, a.twx);
     C* const buf = lds + l * LE;
+    // the planes are row-tiled (ma_tile_off): per kx this block's 2G rows are
+    // one run, and consecutive kx are adjacent within a tile
+    const int ts = a.yt_log2;
     C qa0[MAXIT], qa1[MAXIT], qb0[MAXIT], qb1[MAXIT], qc0[MAXIT], qc1[MAXIT];
-    ma_load<R, N, E, MAXIT>(a.Pxx, pitch, y0, G, total, qa0, qa1);
-    ma_load<R, N, E, MAXIT>(a.Pyy, pitch, y0, G, total, qb0, qb1);
-    ma_load<R, N, E, MAXIT>(a.Pxy, pitch, y0, G, total, qc0, qc1);
+    ma_load_tile<R, N, E, MAXIT>(a.Pxx, a.ncols, ts, y0, G, total, qa0, qa1);
+    ma_load_tile<R, N, E, MAXIT>(a.Pyy, a.ncols, ts, y0, G, total, qb0, qb1);
+    ma_load_tile<R, N, E, MAXIT>(a.Pxy, a.ncols, ts, y0, G, total, qc0, qc1);
     C w3[3][E];
     ma_scatter<R, N, E, MAXIT>(lds, G, total, qa0, qa1, 1);
     F::template run_from_lds<+1>(w3[0], buf, t, tw);
@@ -130,7 +150,16 @@ __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
 #pragma unroll
     for (int m = 0; m < E; ++m) buf[lpad(t + T * m)] = v[m];
     __syncthreads();
-    store_split_stride<R, N, E>(lds, G, 1, a.ncols, a.Dx, a.Dx, pitch, y0, y0 + G);
+    // D's x-spectrum into the row-tiled output plane, rows A = y0 + l and
+    // B = y0 + G + l
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int kx = idx / G, ll = idx - kx * G;
+        const C* bl = lds + ll * LE;
+        const C z1 = bl[lpad(kx)];
+        const C z2 = cconj(bl[lpad((N - kx) & (N - 1))]);
+        a.Dx[ma_tile_off(y0 + ll, kx, a.ncols, ts)] = C{(z1.x + z2.x) * (R)0.5, (z1.y + z2.y) * (R)0.5};
+        a.Dx[ma_tile_off(y0 + G + ll, kx, a.ncols, ts)] = C{(z1.y - z2.y) * (R)0.5, -(z1.x - z2.x) * (R)0.5};
+    }
 }
 
 // Adjoint row pass: λ (x-spectral) -> physical; the products Φyy λ, Φxx λ,

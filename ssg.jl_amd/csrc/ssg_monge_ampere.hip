@@ -288,6 +288,20 @@ int ma_cols_ept_of(ssg_ctx* c) {
     return (e && ept_valid(c->ny, e)) ? e : ept_of<R>(c, 1, c->ny);
 }
 
+// log2 of the row tile of the Monge-Ampère derivative and D planes: the 2G
+// rows one k_ma_rows block of path R owns (its line-pair count G follows
+// from its elements per thread, as in ma_rows)
+template <typename R>
+int ma_tile_log2(ssg_ctx* c) {
+    // at least 8 rows (64 B runs per kx in FP32, 128 B in FP64), whatever a
+    // row-pass block covers; a block of more rows gets a tile of its own
+    const int g = lines_per_block<R>(c->nx, ma_ept_of<R>(c), c->nz * c->ny / 2, 1);
+    const int yt = std::max(8, 2 * g);
+    int s = 0;
+    while ((1 << s) < yt) ++s;
+    return s;
+}
+
 template <typename R>
 int ma_rows(ssg_ctx* c, cpx<R>* const (&planes)[3]) {
     PathBuffers<R>& B = bufs<R>(c);
@@ -300,7 +314,7 @@ int ma_rows(ssg_ctx* c, cpx<R>* const (&planes)[3]) {
         const int G = lines_per_block<R>(N, E, rows / 2, 1);
         const int nblk = rows / (2 * G);
         MaRowsArgs<R> a{planes[0], planes[1], planes[2], planes[0], {nullptr, nullptr, nullptr}, c->ncols_full, G, nblk,
-                        B.twx, c->g3};
+                        B.twx, c->g3, ma_tile_log2<R>(c)};
         const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<R>);
         const double bytes = 4.0 * c->ncols_full * rows * sizeof(cpx<R>);
         int slot, rc;
@@ -357,6 +371,7 @@ int ma_tune_cols(ssg_ctx* c, const cpx<R>* phi) {
     da.outC = pl[2];
     da.scale = (R)1;
     da.nt = ma_stream_planes(c);
+    da.yt_log2 = ma_tile_log2<R>(c);
     ColsArgs<R> ra{};
     ra.inA = pl[0];
     ra.outA = pl[1];
@@ -365,6 +380,7 @@ int ma_tune_cols(ssg_ctx* c, const cpx<R>* phi) {
     ra.body = 1;
     ra.partial = c->ma_partial;
     ra.nt = da.nt;
+    ra.yt_log2 = da.yt_log2;
     const int cands[3] = {4, 8, 16};
     int best = 0, rc = SSG_OK;
     double best_ms = 1e30;
@@ -516,6 +532,7 @@ int ma_eval(ssg_ctx* c, const ssg_ma_params* p, const cpx<R>* phi, cpx<R>* const
     a.outC = planes[2];
     a.scale = (R)scale;
     a.nt = ma_stream_planes(c);
+    a.yt_log2 = ma_tile_log2<R>(c);
     int rc;
     if ((rc = launch_cols_args<R, OP_MA_DERIV>(c, a, lines, c->nz, ma_cols_ept_of<R>(c)))) return rc;
     if ((rc = ma_rows<R>(c, planes))) return rc;
@@ -527,6 +544,7 @@ int ma_eval(ssg_ctx* c, const ssg_ma_params* p, const cpx<R>* phi, cpx<R>* const
     b.body = p->variant == SSG_MA_BODY ? 1 : 0;
     b.partial = c->ma_partial;
     b.nt = a.nt;
+    b.yt_log2 = a.yt_log2;
     if ((rc = launch_cols_args<R, OP_MA_DIFF>(c, b, lines, c->nz, ma_cols_ept_of<R>(c)))) return rc;
     double r2 = 0, h2 = 0;
     if ((rc = ma_read_sums(c, &r2, &h2))) return rc;
@@ -1282,6 +1300,7 @@ int ma_phys_derivs(ssg_ctx* c) {
     a.outB = c->ma_p[1];
     a.outC = c->ma_p[2];
     a.scale = 1.0 / ((double)c->nx * c->ny);
+    a.yt_log2 = ma_tile_log2<double>(c);
     int rc = launch_cols_args<double, OP_MA_DERIV>(c, a, lines, c->nz);
     if (rc) return rc;
     const int ept = ma_ept_of<double>(c);
@@ -1293,7 +1312,8 @@ int ma_phys_derivs(ssg_ctx* c) {
         const int G = lines_per_block<double>(N, E, rows / 2, 1);
         const int nblk = rows / (2 * G);
         MaRowsArgs<double> m{c->ma_p[0], c->ma_p[1], c->ma_p[2], nullptr,
-                             {c->adjc_pd[0], c->adjc_pd[1], c->adjc_pd[2]}, c->ncols_full, G, nblk, D.twx, c->g3};
+                             {c->adjc_pd[0], c->adjc_pd[1], c->adjc_pd[2]}, c->ncols_full, G, nblk, D.twx, c->g3,
+                             ma_tile_log2<double>(c)};
         const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<double>);
         hipLaunchKernelGGL((k_ma_rows<double, N, E, MA_ROWS_PHYS>), dim3(nblk), dim3(G * T), lds, c->stream, m);
         c->launches++;
@@ -1703,6 +1723,7 @@ int omega_w(ssg_ctx* c, double eps) {
     a.outB = c->ma_p[1];
     a.outC = c->ma_p[2];
     a.scale = scale;
+    a.yt_log2 = ma_tile_log2<double>(c);
     int rc = launch_cols_args<double, OP_MA_DERIV>(c, a, lines, c->nz);
     if (rc) return rc;
     rc = with_size_ept(c->nx, ma_ept_of<double>(c), [&](auto tag, auto etag) -> int {
@@ -1712,7 +1733,7 @@ int omega_w(ssg_ctx* c, double eps) {
         const int G = lines_per_block<double>(N, E, rows / 2, 1);
         const int nblk = rows / (2 * G);
         MaRowsArgs<double> m{c->ma_p[0], c->ma_p[1], c->ma_p[2], nullptr, {c->om_pd[0], c->om_pd[1], c->om_pd[2]},
-                             c->ncols_full, G, nblk, D.twx, c->g3};
+                             c->ncols_full, G, nblk, D.twx, c->g3, ma_tile_log2<double>(c)};
         const size_t lds = (size_t)G * lds_line_elems(N) * sizeof(cpx<double>);
         hipLaunchKernelGGL((k_ma_rows<double, N, E, MA_ROWS_PHYS>), dim3(nblk), dim3(G * T), lds, c->stream, m);
         c->launches++;
