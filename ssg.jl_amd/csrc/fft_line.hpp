@@ -93,6 +93,10 @@ template <typename R, int DIR> struct SmallDFT<R, 1, DIR> {
 
 __host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 
+// log2 of a power of two (the lines per block G of every row and column
+// pass): index arithmetic by shifts and masks, not integer division
+__device__ __forceinline__ int gsh(int g) { return __builtin_ctz((unsigned)g); }
+
 // LDS index padding: one spare slot every 2^SSG_LDS_PAD_SHIFT entries keeps
 // the stride-E writes of the first stage off a single bank group.
 #ifndef SSG_LDS_PAD_SHIFT

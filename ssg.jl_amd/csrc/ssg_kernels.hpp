@@ -176,7 +176,7 @@ __device__ __forceinline__ void store_split(cpx<R>* lds, int G, int ncols, cpx<R
     constexpr int LE = lds_line_elems(N);
     const int total = ncols * G;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int kx = idx / G, l = idx - kx * G;
+        const int kx = idx >> gsh(G), l = idx & (G - 1);
         const C* buf = lds + l * LE;
         const C z1 = buf[lpad(kx)];
         const C z2 = cconj(buf[lpad((N - kx) & (N - 1))]);
@@ -196,7 +196,7 @@ __device__ __forceinline__ void store_split_stride(cpx<R>* lds, int G, int lstri
     constexpr int LE = lds_line_elems(N);
     const int total = ncols * G;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int kx = idx / G, l = idx - kx * G;
+        const int kx = idx >> gsh(G), l = idx & (G - 1);
         const C* buf = lds + l * lstride * LE;
         const C z1 = buf[lpad(kx)];
         const C z2 = cconj(buf[lpad((N - kx) & (N - 1))]);
@@ -782,7 +782,7 @@ __device__ __forceinline__ void rows_inv_body(const RowsInvArgs<R>& a, const int
         for (int it = 0; it < MAXIT; ++it) {
             const int idx = threadIdx.x + it * (int)blockDim.x;
             if (idx < total) {
-                const int kx = idx / G, ll = idx - kx * G;
+                const int kx = idx >> gsh(G), ll = idx & (G - 1);
                 const size_t o = (size_t)kx * pitch + y0 + ll;
                 if (MODE == MODE_PAIR) {
                     q0[it] = a.P0[o];
@@ -797,7 +797,7 @@ __device__ __forceinline__ void rows_inv_body(const RowsInvArgs<R>& a, const int
         for (int it = 0; it < MAXIT; ++it) {
             const int idx = threadIdx.x + it * (int)blockDim.x;
             if (idx < total) {
-                const int kx = idx / G, ll = idx - kx * G;
+                const int kx = idx >> gsh(G), ll = idx & (G - 1);
                 C p0 = q0[it], p1 = q1[it];
                 C* bl = lds + ll * LE;
                 const bool edge = (kx == 0) || (2 * kx == N);
@@ -814,7 +814,7 @@ __device__ __forceinline__ void rows_inv_body(const RowsInvArgs<R>& a, const int
         if (gap > 0) {
             const int tz = gap * G;
             for (int idx = threadIdx.x; idx < tz; idx += blockDim.x) {
-                const int q = idx / G, ll = idx - q * G;
+                const int q = idx >> gsh(G), ll = idx & (G - 1);
                 lds[ll * LE + lpad(nc + q)] = C{0, 0};
             }
         }

@@ -37,7 +37,7 @@ __device__ __forceinline__ void ma_load(const cpx<R>* P, int pitch, int y0, int 
     for (int it = 0; it < MAXIT; ++it) {
         const int idx = threadIdx.x + it * (int)blockDim.x;
         if (idx < total) {
-            const int kx = idx / G, ll = idx - kx * G;
+            const int kx = idx >> gsh(G), ll = idx & (G - 1);
             const size_t o = (size_t)kx * pitch + y0 + ll;
             q0[it] = P[o];
             q1[it] = P[o + G];
@@ -54,7 +54,7 @@ __device__ __forceinline__ void ma_load_tile(const cpx<R>* P, int kxn, int s, in
     for (int it = 0; it < MAXIT; ++it) {
         const int idx = threadIdx.x + it * (int)blockDim.x;
         if (idx < total) {
-            const int kx = idx / G, ll = idx - kx * G;
+            const int kx = idx >> gsh(G), ll = idx & (G - 1);
             q0[it] = P[ma_tile_off(y0 + ll, kx, kxn, s)];
             q1[it] = P[ma_tile_off(y0 + G + ll, kx, kxn, s)];
         }
@@ -70,7 +70,7 @@ __device__ __forceinline__ void ma_scatter(cpx<R>* lds, int G, int total, const 
     for (int it = 0; it < MAXIT; ++it) {
         const int idx = threadIdx.x + it * (int)blockDim.x;
         if (idx < total) {
-            const int kx = idx / G, ll = idx - kx * G;
+            const int kx = idx >> gsh(G), ll = idx & (G - 1);
             C p0 = q0[it], p1 = q1[it];
             C* bl = lds + ll * lstride * LE;
             const bool edge = (kx == 0) || (2 * kx == N);
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
     // D's x-spectrum into the row-tiled output plane, rows A = y0 + l and
     // B = y0 + G + l
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int kx = idx / G, ll = idx - kx * G;
+        const int kx = idx >> gsh(G), ll = idx & (G - 1);
         const C* bl = lds + ll * LE;
         const C z1 = bl[lpad(kx)];
         const C z2 = cconj(bl[lpad((N - kx) & (N - 1))]);
