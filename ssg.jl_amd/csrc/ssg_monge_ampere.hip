@@ -485,7 +485,9 @@ int ma_tune(ssg_ctx* c) {
 // stream, so it resumes as soon as the sums land.  Every 1024 polls it
 // queries the stream: an error ends the wait, and a drained stream without
 // the sums is an error, so the wait always ends.
-static int ma_read_sums(ssg_ctx* c, double* r2, double* h2) {
+// `ahead` (may be null) enqueues work behind the reduction before the host
+// waits for it, so the device does not idle through the host's decision.
+static int ma_read_sums(ssg_ctx* c, double* r2, double* h2, int (*ahead)(ssg_ctx*) = nullptr) {
     MaSum* h = c->ma_sum_host;
     __atomic_store_n(&h->r2, kMaSumUnset, __ATOMIC_RELAXED);
     __atomic_store_n(&h->h2, kMaSumUnset, __ATOMIC_RELAXED);
@@ -493,6 +495,10 @@ static int ma_read_sums(ssg_ctx* c, double* r2, double* h2) {
     hipLaunchKernelGGL(k_ma_sum, dim3(1), dim3(kMaSumThreads), 0, c->stream, (const double2*)c->ma_partial,
                        c->ma_partial_used, c->ma_sum_dev);
     HIP_OK(c, hipGetLastError());
+    if (ahead) {
+        const int rc = ahead(c);
+        if (rc) return rc;
+    }
     unsigned long long a = kMaSumUnset, b = kMaSumUnset;
     auto ready = [&]() {
         a = __atomic_load_n(&h->r2, __ATOMIC_ACQUIRE);
@@ -522,7 +528,7 @@ static int ma_read_sums(ssg_ctx* c, double* r2, double* h2) {
 // ‖r‖ / ‖b + εD‖.
 template <typename R>
 int ma_eval(ssg_ctx* c, const ssg_ma_params* p, const cpx<R>* phi, cpx<R>* const (&planes)[3], cpx<R>* dh,
-            double* rabs, double* rrel) {
+            double* rabs, double* rrel, int (*ahead)(ssg_ctx*) = nullptr) {
     const double scale = 1.0 / ((double)c->nx * c->ny);
     const int lines = c->ncols_full * c->nz;
     ColsArgs<R> a{};
@@ -547,7 +553,7 @@ int ma_eval(ssg_ctx* c, const ssg_ma_params* p, const cpx<R>* phi, cpx<R>* const
     b.yt_log2 = a.yt_log2;
     if ((rc = launch_cols_args<R, OP_MA_DIFF>(c, b, lines, c->nz, ma_cols_ept_of<R>(c)))) return rc;
     double r2 = 0, h2 = 0;
-    if ((rc = ma_read_sums(c, &r2, &h2))) return rc;
+    if ((rc = ma_read_sums(c, &r2, &h2, ahead))) return rc;
     // Parseval: sum of r² over the grid = (1/(nx ny)) sum of |r̂|² per level
     *rabs = std::sqrt(r2 * scale);
     *rrel = h2 > 0 ? std::sqrt(r2 / h2) : std::sqrt(r2 * scale);
@@ -840,8 +846,13 @@ static int ma_solve_stage32(ssg_ctx* c, const double* bs, const ssg_ma_params* p
     cpx<float>* const pl[3] = {(cpx<float>*)c->ma_p[0], (cpx<float>*)c->ma_p[1], (cpx<float>*)c->ma_p[2]};
     if ((rc = ma_tridiag<float, float>(c, p, 1, c->ma32_d, c->ma32_phi))) return rc;
     double best = 1e300, ra = 0, rr = 0;
+    // the velocities of this iterate are enqueued behind each evaluation's
+    // residual reduction, ahead of the host's decision: the iterate is the
+    // solution whenever that evaluation converges (almost always the first),
+    // and a later iterate enqueues its own
+    auto surface_ahead = [](ssg_ctx* cc) -> int { return ma_surface_w<float>(cc, cc->ma32_phi); };
     for (int it = 0;; ++it) {
-        if ((rc = ma_eval<float>(c, p, c->ma32_phi, pl, c->ma32_d, &ra, &rr))) return rc;
+        if ((rc = ma_eval<float>(c, p, c->ma32_phi, pl, c->ma32_d, &ra, &rr, +surface_ahead))) return rc;
         d->final_residual = ra;
         d->relative_residual = rr;
         if (it < 64) d->history[it] = rr;
@@ -866,7 +877,7 @@ static int ma_solve_stage32(ssg_ctx* c, const double* bs, const ssg_ma_params* p
     }
     c->ma_last_r0 = d->history[0];
     if (p->warm >= 2 && (rc = hist_push(c, c->ma_hist32, c->ma32_d, c->ma_period))) return rc;
-    return ma_surface_w<float>(c, c->ma32_phi);
+    return SSG_OK;  // (u, v) of the converged iterate are already enqueued
 }
 
 // The coupled stepper's solve for the buoyancy bs.  With FP32 stage solves

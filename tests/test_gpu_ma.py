@@ -721,3 +721,36 @@ def test_coupled_stage_fp32_rejects(gpu):
     assert d.lib.ssg_set_coupling(d.handle, ctypes.byref(p)) == gpu.SSG_ERR_ARG
     gpu.set_coupling(d, eps=0.1, stage_fp32=True)
     d.close()
+
+
+def test_coupled_stage_fp32_full_size(gpu):
+    """The precision configuration's grid, 1024²×32, ε = 0.1, dt = 2.5e-4:
+    60 coupled RK3 steps (two calls) with FP32 stage solves against the FP64
+    coupled step from the same cold solve.  b, φₛ and the surface kinetic
+    energy stay within 1e-5, the total buoyancy is conserved to roundoff, and
+    the solve that ends each call converges to the FP64 tolerance."""
+    import ctypes
+    n, nz, eps, dt = 1024, 32, 0.1, 2.5e-4
+    b0 = smooth_b(n, 0, 6.0)
+    out = {}
+    for prec, s32 in ((gpu.SSG_FP64, False), (gpu.SSG_FP32_MIXED, True)):
+        d = gpu.make_domain(n, n, nz, precision=prec)
+        f = gpu.allocate_fields(d)
+        f.bs = b0
+        gpu.set_coupling(d, eps=eps, tol=1e-10, maxiter=60, variant=gpu.MA_BODY, warm=2, stage_fp32=s32)
+        st = gpu.TimeState(0.0, f)
+        p = gpu.TimeParams(dt, scheme=gpu.RK3, filter_freq=10, filter_strength=36.0, filter_order=8, filter_cutoff=0.65)
+        for k in (30, 30):
+            gpu.timestep(f, d, p, st, k)
+            last = gpu._MaDiag()
+            d.lib.ssg_last_ma_diag(d.handle, ctypes.byref(last))
+            assert last.converged and last.relative_residual <= 1e-10
+        out[s32] = (f.bs, f.phis, gpu.diagnostics(d)["ke_surface"])
+        d.close()
+    (b32, p32, k32), (b64, p64, k64) = out[True], out[False]
+    eb = np.linalg.norm(b32 - b64) / np.linalg.norm(b64)
+    ep = np.linalg.norm(p32 - p64) / np.linalg.norm(p64)
+    ek = abs(k32 - k64) / abs(k64)
+    print(f"1024^2x32 stage_fp32 vs fp64 after 60 steps: b {eb:.2e} phis {ep:.2e} ke {ek:.2e}", flush=True)
+    assert 0 < eb < 1e-5 and ep < 1e-5 and ek < 1e-5
+    assert abs(b32.sum() - b0.sum()) < 1e-9 * np.abs(b0).sum()
