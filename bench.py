@@ -219,7 +219,7 @@ def cpu_c1(inputs, steps):
             "b_rel_l2_gpu_vs_cpu": rel_l2(b_gpu, b)}
 
 
-def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2, step_tol=1e-10, stage_fp32=False):
+def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2, step_tol=1e-10, stage_fp32=False, scheme=None):
     """Coupled step (Monge-Ampère solve at every RK stage, body-forced
     variant, ε = 0.1, relative residual `step_tol`; with `stage_fp32` the
     solves inside a call run in FP32 and the call's last solve in FP64):
@@ -236,8 +236,8 @@ def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2, step_tol=1e-10
             "absolute_residual": diag["final_residual"], "converged": diag["converged"]}
     ssg.set_coupling(d, eps=0.1, tol=step_tol, maxiter=60, variant=ssg.MA_BODY, warm=warm, stage_fp32=stage_fp32)
     st = ssg.TimeState(0.0, f)
-    params = ssg.TimeParams(dt, scheme=ssg.RK3, filter_freq=10, filter_strength=36.0, filter_order=8,
-                            filter_cutoff=0.65)
+    params = ssg.TimeParams(dt, scheme=ssg.RK3 if scheme is None else scheme, filter_freq=10, filter_strength=36.0,
+                            filter_order=8, filter_cutoff=0.65)
     ssg.timestep(f, d, params, st, 1)
     d.sync()
     t0 = time.perf_counter()
@@ -492,7 +492,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None)
-    ap.add_argument("--coupled-steps", type=int, default=200,
+    ap.add_argument("--coupled-ab2-steps", type=int, default=1000,
+                    help="coupled AB2 steps per form (the precision configuration); 0 = skip")
+    ap.add_argument("--coupled-steps", type=int, default=1000,
                     help="also time this many coupled steps (Monge-Ampère solve per stage); 0 = skip")
     ap.add_argument("--ma-warm", type=int, default=2, choices=(0, 1, 2),
                     help="warm start of the coupled step's solves: 0 cold, 1 last D(Φ), 2 extrapolated")
@@ -617,6 +619,26 @@ def main():
         out["coupled_fp32_strict_errors"] = errs(f32s, f64, args.coupled_steps + 1)
         out["roofline_coupled"] = dict(c32["roofline"], form="fp32 stage solves")
         out["roofline_coupled_fp64"] = dict(c64["roofline"], form="fp64")
+        # Mode C at a glance: the metric's grid with the Monge-Ampère solve at
+        # every RK stage, the precision form against the FP64 GPU step
+        out["mode_c"] = {"workload": f"coupled RK3 {nx}x{ny}x{nz} (Monge-Ampere solve per stage, body-forced, "
+                                     f"eps=0.1, tol 1e-10), dt={args.dt}",
+                         "fp32_steps_per_s": c32["steps_per_s"], "fp64_steps_per_s": c64["steps_per_s"],
+                         "fp32_over_fp64": c32["steps_per_s"] / c64["steps_per_s"],
+                         "b_rel_l2_fp32_vs_fp64": out["coupled_fp32_errors"]["b_rel_l2"],
+                         "steps": args.coupled_steps, "roofline": out["roofline_coupled"]}
+        # the precision configuration as specified: coupled AB2, dt = 2.5e-4,
+        # 1000 steps, the precision form against the FP64 GPU step
+        if args.coupled_ab2_steps > 0:
+            log(f"bench: coupled AB2 {nx}x{ny}x{nz}, {args.coupled_ab2_steps} steps per form")
+            a64, g64 = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_ab2_steps, ssg.SSG_FP64,
+                                       args.ma_warm, scheme=ssg.AB2_LowStorage)
+            a32, g32 = measure_coupled(ssg, b0, nx, ny, nz, args.dt, args.coupled_ab2_steps, ssg.SSG_FP32_MIXED,
+                                       args.ma_warm, stage_fp32=True, scheme=ssg.AB2_LowStorage)
+            out["coupled_ab2"] = {"fp64_steps_per_s": a64["steps_per_s"], "fp32_steps_per_s": a32["steps_per_s"],
+                                  "fp32_over_fp64": a32["steps_per_s"] / a64["steps_per_s"],
+                                  "errors": errs(g32, g64, args.coupled_ab2_steps + 1),
+                                  "steps": args.coupled_ab2_steps, "form": "fp32 stage solves, fp64 solve of the returned b"}
     if args.adjoint_steps > 0 and not args.only:
         log(f"bench: adjoint configuration 256x256x16, {args.adjoint_steps} steps")
         out["adjoint"] = measure_adjoint(ssg, args.adjoint_steps)
