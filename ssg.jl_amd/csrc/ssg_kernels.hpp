@@ -434,8 +434,16 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
     const size_t mline = ((size_t)c * a.nlev + mlev) * pitch;
     long toff[MA_PLANES ? E : 1];
     if constexpr (MA_PLANES) {
+        const long r0 = (long)mlev * pitch + t;
+        if ((T >> a.yt_log2) > 0) {
+            // T rows are whole tiles: element m sits T * kxn entries after element 0
+            const long o0 = ma_tile_off(r0, c, kxn, a.yt_log2);
 #pragma unroll
-        for (int m = 0; m < E; ++m) toff[m] = ma_tile_off((long)mlev * pitch + t + T * m, c, kxn, a.yt_log2);
+            for (int m = 0; m < E; ++m) toff[m] = o0 + (long)m * T * kxn;
+        } else {
+#pragma unroll
+            for (int m = 0; m < E; ++m) toff[m] = ma_tile_off(r0 + T * m, c, kxn, a.yt_log2);
+        }
     }
     C* const buf1[1] = {lds + (l * NBUF) * LE};
     typename F::Twiddles tw;

@@ -46,17 +46,20 @@ __device__ __forceinline__ void ma_load(const cpx<R>* P, int pitch, int y0, int 
 }
 
 // the same from a row-tiled plane (ma_tile_off, tiles of 2^s rows of kxn
-// columns): rows A = y0 + l and B = y0 + G + l
+// columns) for a block whose 2G rows y0.. lie in one tile (2G <= 2^s, y0 a
+// multiple of 2G): `tp` points at the block's first row in its tile
+// (ma_tile_off(y0, 0, ...)), so row y0 + j of column kx is tp[(kx << s) + j];
+// rows A = y0 + l and B = y0 + G + l
 template <typename R, int N, int E, int MAXIT>
-__device__ __forceinline__ void ma_load_tile(const cpx<R>* P, int kxn, int s, int y0, int G, int total,
-                                             cpx<R> (&q0)[MAXIT], cpx<R> (&q1)[MAXIT]) {
+__device__ __forceinline__ void ma_load_tile(const cpx<R>* tp, int s, int G, int total, cpx<R> (&q0)[MAXIT],
+                                             cpx<R> (&q1)[MAXIT]) {
 #pragma unroll
     for (int it = 0; it < MAXIT; ++it) {
         const int idx = threadIdx.x + it * (int)blockDim.x;
         if (idx < total) {
-            const int kx = idx >> gsh(G), ll = idx & (G - 1);
-            q0[it] = P[ma_tile_off(y0 + ll, kx, kxn, s)];
-            q1[it] = P[ma_tile_off(y0 + G + ll, kx, kxn, s)];
+            const unsigned o = ((unsigned)(idx >> gsh(G)) << s) + (unsigned)(idx & (G - 1));
+            q0[it] = tp[o];
+            q1[it] = tp[o + G];
         }
     }
 }
@@ -113,10 +116,11 @@ __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
     // the planes are row-tiled (ma_tile_off): per kx this block's 2G rows are
     // one run, and consecutive kx are adjacent within a tile
     const int ts = a.yt_log2;
+    const size_t t0 = (size_t)ma_tile_off(y0, 0, a.ncols, ts);  // this block's rows, one tile
     C qa0[MAXIT], qa1[MAXIT], qb0[MAXIT], qb1[MAXIT], qc0[MAXIT], qc1[MAXIT];
-    ma_load_tile<R, N, E, MAXIT>(a.Pxx, a.ncols, ts, y0, G, total, qa0, qa1);
-    ma_load_tile<R, N, E, MAXIT>(a.Pyy, a.ncols, ts, y0, G, total, qb0, qb1);
-    ma_load_tile<R, N, E, MAXIT>(a.Pxy, a.ncols, ts, y0, G, total, qc0, qc1);
+    ma_load_tile<R, N, E, MAXIT>(a.Pxx + t0, ts, G, total, qa0, qa1);
+    ma_load_tile<R, N, E, MAXIT>(a.Pyy + t0, ts, G, total, qb0, qb1);
+    ma_load_tile<R, N, E, MAXIT>(a.Pxy + t0, ts, G, total, qc0, qc1);
     C w3[3][E];
     ma_scatter<R, N, E, MAXIT>(lds, G, total, qa0, qa1, 1);
     F::template run_from_lds<+1>(w3[0], buf, t, tw);
@@ -152,13 +156,15 @@ __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
     __syncthreads();
     // D's x-spectrum into the row-tiled output plane, rows A = y0 + l and
     // B = y0 + G + l
+    C* const dt0 = a.Dx + t0;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
         const int kx = idx >> gsh(G), ll = idx & (G - 1);
         const C* bl = lds + ll * LE;
         const C z1 = bl[lpad(kx)];
         const C z2 = cconj(bl[lpad((N - kx) & (N - 1))]);
-        a.Dx[ma_tile_off(y0 + ll, kx, a.ncols, ts)] = C{(z1.x + z2.x) * (R)0.5, (z1.y + z2.y) * (R)0.5};
-        a.Dx[ma_tile_off(y0 + G + ll, kx, a.ncols, ts)] = C{(z1.y - z2.y) * (R)0.5, -(z1.x - z2.x) * (R)0.5};
+        const unsigned o = ((unsigned)kx << ts) + (unsigned)ll;
+        dt0[o] = C{(z1.x + z2.x) * (R)0.5, (z1.y + z2.y) * (R)0.5};
+        dt0[o + G] = C{(z1.y - z2.y) * (R)0.5, -(z1.x - z2.x) * (R)0.5};
     }
 }
 
