@@ -465,23 +465,33 @@ __global__ __launch_bounds__(128) void k_ma_tridiag_adj(MaTriAdjArgs a) {
     auto Dk = [&](int k) -> double { return k == 0 ? D0 : (k == nz - 1 ? DN : cf[3 * k + 1] + hs); };
     auto Lk = [&](int k) -> double { return k == nz - 1 ? LN : cf[3 * k]; };       // 1 <= k <= nz-1
     auto Uk = [&](int k) -> double { return k == 0 ? U0 : cf[3 * k + 2]; };        // 0 <= k <= nz-2
-    auto yk = [&](int k) -> double2 {
-        double2 y = {0.0, 0.0};
-        if (a.use_q && k >= 1 && k <= nz - 2) {
-            const cpx<double> qq = a.q[base + (size_t)k * ny];
-            y.x = a.eps * qq.x;
-            y.y = a.eps * qq.y;
+    // Thomas on A'ᵀ: row k has sub U_{k-1}, diagonal D_k, super L_{k+1}.
+    // dp[k] first holds the right-hand side y_k, every value loaded before
+    // the sweep in one straight run (levels clamped into the plane, values
+    // off the interior levels unused), then the sweep overwrites it in place.
+    double cp[NZMAX];
+    double2 dp[NZMAX];
+    if (a.use_q) {
+#pragma unroll
+        for (int k = 0; k < NZMAX; ++k) {
+            const int kk = k > nz - 1 ? nz - 1 : k;
+            const cpx<double> qq = a.q[base + (size_t)kk * ny];
+            const bool in = k >= 1 && k <= nz - 2;
+            dp[k] = {in ? a.eps * qq.x : 0.0, in ? a.eps * qq.y : 0.0};
         }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NZMAX; ++k) dp[k] = {0.0, 0.0};
+    }
+    const cpx<double> pb = a.phibar[(size_t)kx * ny + ky];
+    auto yk = [&](int k) -> double2 {
+        double2 y = dp[k];
         if (k == nz - 1) {
-            const cpx<double> pb = a.phibar[(size_t)kx * ny + ky];
             y.x += pb.x;
             y.y += pb.y;
         }
         return y;
     };
-    // Thomas on A'ᵀ: row k has sub U_{k-1}, diagonal D_k, super L_{k+1}
-    double cp[NZMAX];
-    double2 dp[NZMAX];
     {
         const double d0 = Dk(0);
         const double2 y0 = yk(0);
@@ -643,9 +653,13 @@ __global__ __launch_bounds__(128) void k_omega_tridiag(OmegaTriArgs a) {
     const size_t base = (size_t)kx * nz * ny + ky;
     double cp[NZMAX];
     double2 dp[NZMAX];
+    // every right-hand side in one straight run of loads (levels clamped into
+    // the plane; positions past the interior are unused)
 #pragma unroll
-    for (int k = 1; k < NZMAX; ++k)
-        if (k < nz - 1) dp[k] = a.F[base + (size_t)k * ny];
+    for (int k = 1; k < NZMAX; ++k) {
+        const int kk = k > nz - 1 ? nz - 1 : k;
+        dp[k] = a.F[base + (size_t)kk * ny];
+    }
     cp[0] = 0.0;
     dp[0] = {0.0, 0.0};
 #pragma unroll

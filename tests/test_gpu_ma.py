@@ -89,9 +89,12 @@ def test_k13_surface_eps0_dense(gpu):
     assert np.abs(phi - ref).max() < 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("nz", [3, 5, 6])
 @pytest.mark.parametrize("variant", [0, 1])
-def test_eps_newton_parity_tiny(gpu, variant):
-    n, nz, eps = 8, 5, 0.1
+def test_eps_newton_parity_tiny(gpu, variant, nz):
+    """Against dense Newton at 8²; nz = 3, 5, 6 cover the per-wavenumber
+    solve's two-lane split at its smallest, odd and even sizes."""
+    n, eps = 8, 0.1
     b = smooth_b(n, 4, 1.2)
     d, f, diag = gpu_solve(gpu, b, nz, eps, variant)
     assert diag["converged"], diag
