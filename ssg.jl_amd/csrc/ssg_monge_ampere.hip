@@ -880,11 +880,6 @@ static int ma_solve_stage32(ssg_ctx* c, const double* bs, const ssg_ma_params* p
     return SSG_OK;  // (u, v) of the converged iterate are already enqueued
 }
 
-// The coupled stepper's solve for the buoyancy bs.  With FP32 stage solves
-// (FP32-mixed context, params->stage_fp32) every solve but the call's last
-// (final_solve) runs in FP32 once a warm start is held; the others, and an
-// FP32 solve that fails, run in FP64 from the FP32 history's guess and then
-// continue that history.
 // SSG_MA_TRACE=1 in the environment prints one line per coupled solve to
 // stderr: FP32 or FP64, evaluations, first and last relative residual.
 static void ma_trace(const char* kind, const ssg_ma_diag& d) {
@@ -894,6 +889,11 @@ static void ma_trace(const char* kind, const ssg_ma_diag& d) {
                 d.iterations_fp32, d.history[0], d.relative_residual, d.converged);
 }
 
+// The coupled stepper's solve for the buoyancy bs.  With FP32 stage solves
+// (FP32-mixed context, params->stage_fp32) every solve but the call's last
+// (final_solve) runs in FP32 once a warm start is held; the others, and an
+// FP32 solve that fails, run in FP64 from the FP32 history's guess and then
+// continue that history.
 int ma_solve_coupled(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_diag* d, bool final_solve) {
     const bool s32 = p->stage_fp32 && c->precision == SSG_FP32_MIXED;
     int rc;
