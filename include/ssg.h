@@ -305,8 +305,9 @@ int ssg_last_timing(ssg_ctx* ctx, double* ms, int64_t* launches);
    input read once, every output written once).  Classes: 0 = row pass
    forward (k_rows_fwd), 1 = column pass (k_cols), 2 = row pass inverse with
    fused stage epilogue (k_rows_inv), 3 = Monge-Ampère row pass (k_ma_rows),
-   4 = Monge-Ampère per-wavenumber solve (k_ma_tridiag); 11 and 12 = the
-   FP32 launches of classes 3 and 4. */
+   4 = Monge-Ampère per-wavenumber solve (k_ma_tridiag), 5 = the same solve
+   forming its extrapolated warm start from the solve history; 11, 12 and 13
+   = the FP32 launches of classes 3, 4 and 5. */
 int ssg_profile(ssg_ctx* ctx, int32_t enable);
 
 /* Autotuned launch shape of the three line-FFT kernel classes for one path

@@ -112,6 +112,7 @@ struct ssg_ctx {
     MaSum* ma_sum_host = nullptr;     // pinned: reduced residual sums written by the device
     MaSum* ma_sum_dev = nullptr;      // device address of ma_sum_host
     int ma_stream = 0;                // nontemporal access to the C3 planes
+    bool ma_ext_fused = true;         // the first solve of a warm start forms its extrapolation (k_ma_tridiag)
     int ma_partial_cap = 0;
     int ma_partial_used = 0;
     bool ma_have_d = false;

@@ -1106,6 +1106,18 @@ static __global__ __launch_bounds__(256) void k_wmax(const double2* __restrict__
     }
 }
 
+// ca x + cb y + cc z of one plane element in FP64 with explicit fused
+// multiply-adds and one rounding: every kernel that forms the quadratic
+// warm-start extrapolation (k_extrap, k_ma_tridiag's extrapolating form)
+// produces the same bits.
+template <typename C>
+__device__ __forceinline__ C extrap3(C x, double ca, C y, double cb, C z, double cc) {
+    using R = typename cpx_real_t<C>::type;
+    const double rx = fma(cc, (double)z.x, fma(cb, (double)y.x, ca * (double)x.x));
+    const double ry = fma(cc, (double)z.y, fma(cb, (double)y.y, ca * (double)x.y));
+    return C{(R)rx, (R)ry};
+}
+
 // warm-start extrapolation: out = ca a + cb b + cc c (c may be null).  One
 // element per thread (grid of ceil(n / 256) blocks); plane access as in
 // ld_plane / st_plane.
@@ -1119,12 +1131,11 @@ __global__ __launch_bounds__(256) void k_extrap(C* __restrict__ out, const C* __
     const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (q >= n) return;
     const C x = ld_plane<R>(a + q, nt), y = ld_plane<R>(b + q, nt);
-    double rx = ca * (double)x.x + cb * (double)y.x, ry = ca * (double)x.y + cb * (double)y.y;
     if (c) {
-        const C z = ld_plane<R>(c + q, nt);
-        rx += cc * (double)z.x;
-        ry += cc * (double)z.y;
+        st_plane<R>(out + q, extrap3(x, ca, y, cb, ld_plane<R>(c + q, nt), cc), nt);
+        return;
     }
+    const double rx = ca * (double)x.x + cb * (double)y.x, ry = ca * (double)x.y + cb * (double)y.y;
     st_plane<R>(out + q, C{(R)rx, (R)ry}, nt);
 }
 

@@ -258,6 +258,11 @@ struct MaTriArgs {
     int use_d;                  // 0: D term omitted (first solve without history)
     int nt;                     // nontemporal plane access (ld_plane)
     Grid g;
+    // extrapolating form: D̂ is the warm start ring[0] rc[0] + ring[1] rc[1]
+    // + ring[2] rc[2] (extrap3), formed here and also stored to Dg
+    const cpx<R>* ring[3];
+    double rc[3];
+    cpx<R>* Dg;
 };
 
 // Two lanes per wavenumber (a twisted factorisation): the even lane
@@ -266,8 +271,12 @@ struct MaTriArgs {
 // same code on mirrored coefficients; the two interface rows are solved
 // after exchanging their reduced forms (lane shuffles), and each lane
 // back-substitutes its own half.  Half the registers and half the dependent
-// chain of one Thomas sweep per wavenumber.  NZH >= ceil(nz/2).
-template <typename R, typename P, int NZH>
+// chain of one Thomas sweep per wavenumber.  NZH >= max(2, ceil(nz/2)).
+// EXT: the extrapolating form, which reads the three history entries in
+// place of D̂, forms the warm start itself and stores it (the residual pass
+// needs it as D̂_prev): one pass over the planes fewer than k_extrap
+// followed by this solve, with the same bits.
+template <typename R, typename P, int NZH, bool EXT = false>
 __global__ __launch_bounds__(256) void k_ma_tridiag(MaTriArgs<R, P> a) {
     // the stencil table (3 nz + 6 values, nz <= 2 NZH) in LDS: its reads
     // cost no vector-memory waits in the sweep
@@ -290,35 +299,61 @@ __global__ __launch_bounds__(256) void k_ma_tridiag(MaTriArgs<R, P> a) {
     auto lev = [&](int j) { return up ? nz - 1 - j : j; };
     double cc[NZH];
     double2 dd[NZH];
-    // every D̂ value the lane needs in one straight run of loads: its interior
-    // rows (dd[j] holds the right-hand side until the sweep overwrites it)
-    // and, in slot 0, the interior row next to its boundary row.  Positions
-    // past this lane's rows load a clamped, in-bounds level and are unused.
+    // every D̂ value the lane needs in one straight run of loads: slot j >= 1
+    // holds level lev(j) (dd[j] holds its right-hand side until the sweep
+    // overwrites it; slot 1 is also the interior row next to the boundary
+    // row); positions past this lane's rows load a clamped, in-bounds level
+    // and are unused.  Slot 0 (the boundary level, which carries no D̂) is
+    // loaded only by the extrapolating form, which stores the warm start on
+    // every level.
     cpx<R> dv[NZH];
-    if (a.use_d) {
-        const cpx<R>* col = a.Dh + base;
+    if (EXT || a.use_d) {
         int lv[NZH];
 #pragma unroll
         for (int j = 0; j < NZH; ++j) {
-            const int k = j == 0 ? (up ? nz - 2 : 1) : lev(j);
+            const int k = lev(j);
             lv[j] = k < 0 ? 0 : (k > nz - 1 ? nz - 1 : k);
         }
+        constexpr int J0 = EXT ? 0 : 1;
         using V = typename vec2_t<R>::type;
-        if (a.nt) {
+        auto load_all = [&](const cpx<R>* col, cpx<R> (&out)[NZH]) {
+            if (a.nt) {
 #pragma unroll
-            for (int j = 0; j < NZH; ++j) {
-                const V w = __builtin_nontemporal_load(reinterpret_cast<const V*>(col + (size_t)lv[j] * ny));
-                dv[j] = cpx<R>{w.x, w.y};
+                for (int j = J0; j < NZH; ++j) {
+                    const V w = __builtin_nontemporal_load(reinterpret_cast<const V*>(col + (size_t)lv[j] * ny));
+                    out[j] = cpx<R>{w.x, w.y};
+                }
+            } else {
+#pragma unroll
+                for (int j = J0; j < NZH; ++j) out[j] = col[(size_t)lv[j] * ny];
+            }
+        };
+        if constexpr (EXT) {
+            cpx<R> x1[NZH], x2[NZH];
+            load_all(a.ring[0] + base, dv);
+            load_all(a.ring[1] + base, x1);
+            load_all(a.ring[2] + base, x2);
+#pragma unroll
+            for (int j = 0; j < NZH; ++j) dv[j] = extrap3(dv[j], a.rc[0], x1[j], a.rc[1], x2[j], a.rc[2]);
+            cpx<R>* const g = a.Dg + base;
+            if (a.nt) {
+#pragma unroll
+                for (int j = 0; j < NZH; ++j)
+                    if (j < cnt)
+                        __builtin_nontemporal_store(V{dv[j].x, dv[j].y}, reinterpret_cast<V*>(g + (size_t)lev(j) * ny));
+            } else {
+#pragma unroll
+                for (int j = 0; j < NZH; ++j)
+                    if (j < cnt) g[(size_t)lev(j) * ny] = dv[j];
             }
         } else {
-#pragma unroll
-            for (int j = 0; j < NZH; ++j) dv[j] = col[(size_t)lv[j] * ny];
+            load_all(a.Dh + base, dv);
         }
     }
     auto rhs_of = [&](int j) -> double2 {
         double2 f = {0.0, 0.0};
         if (a.body) f = b2;
-        if (a.use_d) {
+        if (EXT || a.use_d) {
             f.x += a.eps * (double)dv[j].x;
             f.y += a.eps * (double)dv[j].y;
         }
@@ -328,7 +363,7 @@ __global__ __launch_bounds__(256) void k_ma_tridiag(MaTriArgs<R, P> a) {
     for (int j = 1; j < NZH; ++j) dd[j] = rhs_of(j);
     // the boundary row of this lane, with the next level eliminated through
     // the adjacent interior row (src/poisson.jl:309-364)
-    const double2 fb = rhs_of(0);
+    const double2 fb = rhs_of(1);
     {
         double diag, next;
         double2 f;

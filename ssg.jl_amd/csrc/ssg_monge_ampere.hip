@@ -70,6 +70,9 @@ int ma_alloc(ssg_ctx* c) {
         c->ma_stream = atoi(env) != 0 ? 1 : 0;
     else
         c->ma_stream = c3 * sizeof(cpx<double>) > kMaStreamBytes ? 1 : 0;
+    // SSG_MA_EXTRAP_FUSED=0: the quadratic warm start in a pass of its own
+    // (k_extrap) before the first solve; the same bits either way
+    if (const char* env = getenv("SSG_MA_EXTRAP_FUSED")) c->ma_ext_fused = atoi(env) != 0;
     HIP_OK(c, hipMalloc(&c->ma_phi, c3 * sizeof(cpx<double>)));
     HIP_OK(c, hipMalloc(&c->ma_dprev, c3 * sizeof(cpx<double>)));
     for (int i = 0; i < 3; ++i) HIP_OK(c, hipMalloc(&c->ma_p[i], c3 * sizeof(cpx<double>)));
@@ -230,9 +233,17 @@ void ma_history_reset(ssg_ctx* c) {
     c->ma_phi_current = false;
 }
 
-// Φ̂ = L⁻¹(b̂ + ε D̂) per wavenumber (D̂ term omitted when use_d = 0)
+// the extrapolating form of k_ma_tridiag holds three history entries per
+// level in registers: up to 64 levels (beyond, it would spill)
+constexpr int kMaExtMaxNz = 64;
+
+// Φ̂ = L⁻¹(b̂ + ε D̂) per wavenumber (D̂ term omitted when use_d = 0); with
+// `ext`, D̂ is the quadratic warm start 3 ext[0] - 3 ext[1] + ext[2], formed
+// in the solve and stored to Dh (the extrapolating form of k_ma_tridiag).
 template <typename R, typename P>
-int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d, const cpx<R>* Dh, cpx<P>* Phi) {
+int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d, const cpx<R>* Dh, cpx<P>* Phi,
+               const cpx<R>* const* ext = nullptr) {
+    if (ext && c->nz > kMaExtMaxNz) return set_err(c, SSG_ERR_ARG, "extrapolating solve with nz > %d", kMaExtMaxNz);
     MaTriArgs<R, P> a{};
     a.Dh = Dh;
     a.B2 = c->ma_b2;
@@ -246,25 +257,55 @@ int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d, const cpx<R>* Dh, 
     a.use_d = use_d;
     a.nt = ma_stream_planes(c);
     a.g = c->g;
+    if (ext) {
+        for (int i = 0; i < 3; ++i) a.ring[i] = ext[i];
+        a.rc[0] = 3.0;
+        a.rc[1] = -3.0;
+        a.rc[2] = 1.0;
+        a.Dg = const_cast<cpx<R>*>(Dh);
+        a.use_d = 1;
+    }
     const long total = (long)c->ncols_full * c->ny;
     // two lanes per wavenumber
     const int nblk = (int)((2 * total + 255) / 256);
-    const double bytes = (double)total * (c->nz * (sizeof(cpx<R>) + sizeof(cpx<P>)) + sizeof(cpx<double>));
+    const double bytes =
+        (double)total * (c->nz * ((ext ? 4 : 1) * sizeof(cpx<R>) + sizeof(cpx<P>)) + sizeof(cpx<double>));
     int slot, rc;
     if ((rc = prof_begin(c, &slot))) return rc;
-    if (c->nz <= 8)
-        launch_on(c, slot, k_ma_tridiag<R, P, 4>, dim3(nblk), dim3(256), 0, a);
-    else if (c->nz <= 16)
-        launch_on(c, slot, k_ma_tridiag<R, P, 8>, dim3(nblk), dim3(256), 0, a);
-    else if (c->nz <= 32)
-        launch_on(c, slot, k_ma_tridiag<R, P, 16>, dim3(nblk), dim3(256), 0, a);
-    else if (c->nz <= 64)
-        launch_on(c, slot, k_ma_tridiag<R, P, 32>, dim3(nblk), dim3(256), 0, a);
-    else
-        launch_on(c, slot, k_ma_tridiag<R, P, 64>, dim3(nblk), dim3(256), 0, a);
+    auto go = [&](auto kern) { launch_on(c, slot, kern, dim3(nblk), dim3(256), 0, a); };
+    if (ext) {  // nz <= kMaExtMaxNz (ma_tridiag_warm)
+        if (c->nz <= 8) go(k_ma_tridiag<R, P, 4, true>);
+        else if (c->nz <= 16) go(k_ma_tridiag<R, P, 8, true>);
+        else if (c->nz <= 32) go(k_ma_tridiag<R, P, 16, true>);
+        else go(k_ma_tridiag<R, P, 32, true>);
+    } else {
+        if (c->nz <= 8) go(k_ma_tridiag<R, P, 4>);
+        else if (c->nz <= 16) go(k_ma_tridiag<R, P, 8>);
+        else if (c->nz <= 32) go(k_ma_tridiag<R, P, 16>);
+        else if (c->nz <= 64) go(k_ma_tridiag<R, P, 32>);
+        else go(k_ma_tridiag<R, P, 64>);
+    }
     c->launches++;
     HIP_OK(c, hipGetLastError());
-    return prof_end(c, slot, std::is_same<R, float>::value ? 12 : 4, bytes);
+    const int kind = ext ? (std::is_same<R, float>::value ? 13 : 5) : (std::is_same<R, float>::value ? 12 : 4);
+    return prof_end(c, slot, kind, bytes);
+}
+
+// The first iterate of a solve warm-started from history h (of `period`
+// solves per step, the working buffer `work`): with three steps of history
+// the quadratic extrapolation is formed inside the solve, otherwise
+// hist_extrapolate runs first (as does no extrapolation without history).
+template <typename R, typename P>
+int ma_tridiag_warm(ssg_ctx* c, const ssg_ma_params* p, const SolveHistoryT<cpx<R>>& h, int period, cpx<R>* work,
+                    cpx<P>* Phi) {
+    const int P3 = period;
+    if (c->ma_ext_fused && c->nz <= kMaExtMaxNz && P3 > 0 && h.n >= 3 * P3) {
+        const cpx<R>* const ext[3] = {h.ring[P3 - 1], h.ring[2 * P3 - 1], h.ring[3 * P3 - 1]};
+        return ma_tridiag<R, P>(c, p, 1, work, Phi, ext);
+    }
+    int rc;
+    if (h.n > 0 && (rc = hist_extrapolate(c, h, work, period))) return rc;
+    return ma_tridiag<R, P>(c, p, 1, work, Phi);
 }
 
 // x-inverse of the three derivative planes, D = Φxx Φyy - Φxy², x-FFT of D
@@ -584,12 +625,17 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
     if (rc0) return rc0;
     const bool warm = p->warm && c->ma_have_d;
     const bool extrap = warm && p->warm >= 2;
+    // A warm solve whose predecessor already started near the FP32 floor
+    // skips the FP32 phase: it could not lower the residual, and its passes
+    // and precision switch would only add to the FP64 iterations
+    const bool phase32 = c->precision == SSG_FP32_MIXED && !(warm && c->ma_last_r0 <= 10.0 * kMaFp32Floor);
     if (!extrap) {
         ma_history_reset(c);  // the last D̂ back in the working buffer
-    } else {
+    } else if (phase32) {
         // extrapolated warm start (with an empty ring the working buffer
         // holds the last D̂, a plain warm start); the residual stays exact,
-        // since the first iterate is L⁻¹ of whatever D̂ it was built from
+        // since the first iterate is L⁻¹ of whatever D̂ it was built from.
+        // Without the FP32 phase the first solve forms it (ma_tridiag_warm).
         if (c->ma_hist.n > 0 && (rc0 = hist_extrapolate(c, c->ma_hist, c->ma_dprev, c->ma_period))) return rc0;
     }
     memset(d, 0, sizeof(*d));
@@ -614,10 +660,6 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
         }
         return it > p->maxiter;
     };
-    // A warm solve whose predecessor already started near the FP32 floor
-    // skips the FP32 phase: it could not lower the residual, and its passes
-    // and precision switch would only add to the FP64 iterations
-    const bool phase32 = c->precision == SSG_FP32_MIXED && !(warm && c->ma_last_r0 <= 10.0 * kMaFp32Floor);
     if (phase32) {
         cpx<float>* const pl[3] = {(cpx<float>*)c->ma_p[0], (cpx<float>*)c->ma_p[1], (cpx<float>*)c->ma_p[2]};
         if (warm) {
@@ -648,7 +690,11 @@ int ma_iterate(ssg_ctx* c, const ssg_ma_params* p, ssg_ma_diag* d) {
         }
     } else {
         if (!warm) HIP_OK(c, hipMemsetAsync(c->ma_dprev, 0, c3 * sizeof(cpx<double>), c->stream));
-        if ((rc = ma_tridiag<double, double>(c, p, warm ? 1 : 0, c->ma_dprev, c->ma_phi))) return rc;
+        if (extrap)
+            rc = ma_tridiag_warm<double, double>(c, p, c->ma_hist, c->ma_period, c->ma_dprev, c->ma_phi);
+        else
+            rc = ma_tridiag<double, double>(c, p, warm ? 1 : 0, c->ma_dprev, c->ma_phi);
+        if (rc) return rc;
     }
     cpx<double>* const pl[3] = {c->ma_p[0], c->ma_p[1], c->ma_p[2]};
     for (;;) {
@@ -840,11 +886,10 @@ static int ma_seed32(ssg_ctx* c, const ssg_ma_params* p) {
 static int ma_solve_stage32(ssg_ctx* c, const double* bs, const ssg_ma_params* p, ssg_ma_diag* d) {
     int rc;
     if ((rc = rfft2_device(c, bs, c->ma_b2))) return rc;
-    if (c->ma_hist32.n > 0 && (rc = hist_extrapolate(c, c->ma_hist32, c->ma32_d, c->ma_period))) return rc;
     memset(d, 0, sizeof(*d));
     const double tol32 = std::max(p->tol, (double)SSG_MA_STAGE_TOL32);
     cpx<float>* const pl[3] = {(cpx<float>*)c->ma_p[0], (cpx<float>*)c->ma_p[1], (cpx<float>*)c->ma_p[2]};
-    if ((rc = ma_tridiag<float, float>(c, p, 1, c->ma32_d, c->ma32_phi))) return rc;
+    if ((rc = ma_tridiag_warm<float, float>(c, p, c->ma_hist32, c->ma_period, c->ma32_d, c->ma32_phi))) return rc;
     double best = 1e300, ra = 0, rr = 0;
     // the velocities of this iterate are enqueued behind each evaluation's
     // residual reduction, ahead of the host's decision: the iterate is the

@@ -343,6 +343,46 @@ def test_extrapolated_warm_start(gpu, prec):
     assert it2 <= it1
 
 
+@pytest.mark.parametrize("scheme", [1, 0])
+@pytest.mark.parametrize("form", ["fp64", "stage_fp32"])
+def test_fused_extrapolation_is_bitwise(gpu, scheme, form, monkeypatch):
+    """The quadratic warm start formed inside the first per-wavenumber solve
+    (k_ma_tridiag's extrapolating form, the default) gives the same bits as
+    the k_extrap pass followed by the plain solve (SSG_MA_EXTRAP_FUSED=0):
+    coupled RK3 and AB2 steps at 64²×8 with warm=2, FP64 and FP32 stage
+    solves, b and φₛ compared after each of two calls.  The profile counts
+    show which form ran."""
+    n, nz, eps, dt, calls = 64, 8, 0.1, 1e-3, (6, 6)
+    b0 = smooth_b(n, 6, 6.0)
+    s32 = form == "stage_fp32"
+    prec = gpu.SSG_FP32_MIXED if s32 else gpu.SSG_FP64
+    kind = 13 if s32 else 5  # profile class of the extrapolating solve
+    res = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("SSG_MA_EXTRAP_FUSED", fused)
+        d = gpu.make_domain(n, n, nz, precision=prec)
+        f = gpu.allocate_fields(d)
+        f.bs = b0
+        gpu.set_coupling(d, eps=eps, tol=1e-10, maxiter=60, variant=gpu.MA_BODY, warm=2, stage_fp32=s32)
+        st = gpu.TimeState(0.0, f)
+        p = gpu.TimeParams(dt, scheme=scheme, filter_freq=5, filter_strength=36.0, filter_order=8, filter_cutoff=0.65)
+        d.profile(True)
+        outs = []
+        for k in calls:
+            gpu.timestep(f, d, p, st, k)
+            outs.append((f.bs, f.phis))
+        launches = d.profile_read(kind)[1]
+        d.profile(False)
+        d.close()
+        res[fused] = (outs, launches)
+    (o1, n1), (o0, n0) = res["1"], res["0"]
+    print(f"{form} scheme {scheme}: extrapolating solves {n1} (fused), {n0} (separate)", flush=True)
+    assert n1 > 0 and n0 == 0
+    for (b1, p1), (b0_, p0) in zip(o1, o0):
+        assert np.array_equal(b1, b0_)
+        assert np.array_equal(p1, p0)
+
+
 def test_warm_start_history_across_mode_changes(gpu):
     """The extrapolated warm start's solve history across every change of
     pattern: coupled RK3 (three solves per step), coupled AB2 (one), a
