@@ -251,7 +251,8 @@ def measure_coupled(ssg, b0, nx, ny, nz, dt, steps, prec, warm=2, step_tol=1e-10
     d.profile(True)
     ssg.timestep(f, d, params, st, 10)
     names = {0: "k_rows_fwd", 1: "k_cols (tendency and MA ops)", 2: "k_rows_inv", 3: "k_ma_rows",
-             4: "k_ma_tridiag", 11: "k_ma_rows (FP32)", 12: "k_ma_tridiag (FP32)"}
+             4: "k_ma_tridiag", 5: "k_ma_tridiag (extrapolating)", 11: "k_ma_rows (FP32)",
+             12: "k_ma_tridiag (FP32)", 13: "k_ma_tridiag (FP32, extrapolating)"}
     kernels = {}
     for k, nm in names.items():
         ms, n, by = d.profile_read(k)
