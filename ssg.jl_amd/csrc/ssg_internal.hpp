@@ -347,9 +347,12 @@ int launch_cols_args(ssg_ctx* c, ColsArgs<R> a, int nlines, int nlev, int ept_fo
         constexpr int E = decltype(etag)::value;
         constexpr int T = M / E;
         const int le = lds_line_elems(M) * (int)sizeof(cpx<R>) * cols_nbuf(OP);
+        // OP_MA_DIFF: nlines = kxn * nlev, blocks of G kx of one level (G <= kxn),
+        // ceil(kxn / G) groups per level
+        const int glim = OP == OP_MA_DIFF ? nlines / nlev : nlines;
         int G = 1;
-        while (2 * G * T <= 256 && 2 * G * le <= kMaxLdsBytes && 2 * G <= nlines) G *= 2;
-        const int nblk = (nlines + G - 1) / G;
+        while (2 * G * T <= 256 && 2 * G * le <= kMaxLdsBytes && 2 * G <= glim) G *= 2;
+        const int nblk = OP == OP_MA_DIFF ? (glim + G - 1) / G * nlev : (nlines + G - 1) / G;
         const double kmx = M_PI * c->nx / c->lx, kmy = M_PI * c->ny / c->ly;
         a.ncols = nlines;
         a.G = G;

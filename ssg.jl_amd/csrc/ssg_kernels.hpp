@@ -419,16 +419,23 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
     C* lds = reinterpret_cast<C*>(smem_raw);
     const int l = threadIdx.x / T, t = threadIdx.x - l * T;
     const int line = bidx * a.G + l;
-    const bool live = line < a.ncols;
     const int pitch = a.g.ny;                         // lines are contiguous runs of ny
     constexpr bool MA_PLANES = (OP == OP_MA_DERIV || OP == OP_MA_DIFF);
-    // Monge-Ampère passes: lines run level-major (line = level * kxn + kx), so
-    // the lines of a block are consecutive kx of one level; lines past the
-    // end stand in for line 0 (in bounds, nothing stored)
+    // Monge-Ampère passes: the lines of a block are G consecutive kx of one
+    // level.  OP_MA_DERIV runs them level-major (line = level * kxn + kx);
+    // OP_MA_DIFF runs blocks kx-group-major (block = group * nlev + level,
+    // kx = group * G + l), so the nlev blocks of a group are consecutive
+    // (k_cols places them on one XCD) and share the forcing columns they
+    // read through that XCD's L2.  Lines past the end stand in for kx 0 of
+    // level 0 (in bounds, nothing stored).
     const int kxn = (a.g.nx >> 1) + 1;
-    const int mlev = MA_PLANES ? (live ? line : 0) / kxn : 0;
-    const int c = MA_PLANES ? (live ? line : 0) - mlev * kxn
-                            : (a.nlev > 1 ? line / a.nlev : line);  // kx column of this line
+    const int dgrp = OP == OP_MA_DIFF ? bidx / a.nlev : 0;
+    const bool live = OP == OP_MA_DIFF ? dgrp * a.G + l < kxn : line < a.ncols;
+    const int mlev = OP == OP_MA_DIFF ? (live ? bidx - dgrp * a.nlev : 0)
+                                      : (MA_PLANES ? (live ? line : 0) / kxn : 0);
+    const int c = OP == OP_MA_DIFF ? (live ? dgrp * a.G + l : 0)
+                                   : (MA_PLANES ? (live ? line : 0) - mlev * kxn
+                                                : (a.nlev > 1 ? line / a.nlev : line));  // kx column of this line
     // offset of this (kx, level) line in the [kx][level][y] planes, and of its
     // elements in the row-tiled planes
     const size_t mline = ((size_t)c * a.nlev + mlev) * pitch;
@@ -690,7 +697,7 @@ __device__ __forceinline__ void cols_body(const ColsArgs<R>& a, const int bidx) 
 
 template <typename R, int M, int E, int OP, bool PP>
 __global__ __launch_bounds__(256) void k_cols(ColsArgs<R> a) {
-    cols_body<R, M, E, OP, PP>(a, blockIdx.x);
+    cols_body<R, M, E, OP, PP>(a, OP == OP_MA_DIFF ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
