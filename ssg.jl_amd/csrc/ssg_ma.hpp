@@ -45,21 +45,54 @@ __device__ __forceinline__ void ma_load(const cpx<R>* P, int pitch, int y0, int 
     }
 }
 
-// the same from a row-tiled plane (ma_tile_off, tiles of 2^s rows of kxn
+// Gather from a row-tiled plane (ma_tile_off, tiles of 2^s rows of kxn
 // columns) for a block whose 2G rows y0.. lie in one tile (2G <= 2^s, y0 a
 // multiple of 2G): `tp` points at the block's first row in its tile
 // (ma_tile_off(y0, 0, ...)), so row y0 + j of column kx is tp[(kx << s) + j];
-// rows A = y0 + l and B = y0 + G + l
-template <typename R, int N, int E, int MAXIT>
-__device__ __forceinline__ void ma_load_tile(const cpx<R>* tp, int s, int G, int total, cpx<R> (&q0)[MAXIT],
-                                             cpx<R> (&q1)[MAXIT]) {
+// rows A = y0 + l and B = y0 + G + l.  The plane holds every column
+// kx = 0..N/2 (the Monge-Ampère planes): E/2 whole passes over kx < N/2,
+// with no bounds test.  The thread of kx = 0 also fetches the Nyquist
+// column's real parts into the imaginary slots of its DC pair: the Hermitian
+// unpack discards both imaginary parts (the DC and Nyquist bins of a real
+// row are real), so no pass and no registers go to the one column past N/2.
+template <typename R, int N, int E>
+__device__ __forceinline__ void ma_load_tile_full(const cpx<R>* tp, int s, int G, cpx<R> (&q0)[E / 2],
+                                                  cpx<R> (&q1)[E / 2]) {
+    constexpr int T = N / E;
 #pragma unroll
-    for (int it = 0; it < MAXIT; ++it) {
-        const int idx = threadIdx.x + it * (int)blockDim.x;
-        if (idx < total) {
-            const unsigned o = ((unsigned)(idx >> gsh(G)) << s) + (unsigned)(idx & (G - 1));
-            q0[it] = tp[o];
-            q1[it] = tp[o + G];
+    for (int it = 0; it < E / 2; ++it) {
+        const int idx = threadIdx.x + it * G * T;
+        const unsigned o = ((unsigned)(idx >> gsh(G)) << s) + (unsigned)(idx & (G - 1));
+        q0[it] = tp[o];
+        q1[it] = tp[o + G];
+    }
+    if ((int)threadIdx.x < G) {
+        const unsigned o = ((unsigned)(N / 2) << s) + threadIdx.x;
+        q0[0].y = tp[o].x;
+        q1[0].y = tp[o + G].x;
+    }
+}
+
+// Hermitian unpack of ma_load_tile_full's registers into the line buffers:
+// line ll receives X = A + iB of its rows A (re) and B (im)
+template <typename R, int N, int E>
+__device__ __forceinline__ void ma_scatter_full(cpx<R>* lds, int G, const cpx<R> (&q0)[E / 2],
+                                                const cpx<R> (&q1)[E / 2]) {
+    using C = cpx<R>;
+    constexpr int T = N / E;
+    constexpr int LE = lds_line_elems(N);
+#pragma unroll
+    for (int it = 0; it < E / 2; ++it) {
+        const int idx = threadIdx.x + it * G * T;
+        const int kx = idx >> gsh(G), ll = idx & (G - 1);
+        const C p0 = q0[it], p1 = q1[it];
+        C* bl = lds + ll * LE;
+        if (it == 0 && kx == 0) {
+            bl[lpad(0)] = C{p0.x, p1.x};
+            bl[lpad(N / 2)] = C{p0.y, p1.y};
+        } else {
+            bl[lpad(kx)] = C{p0.x - p1.y, p0.y + p1.x};
+            bl[lpad(N - kx)] = C{p0.x + p1.y, p1.x - p0.y};
         }
     }
 }
@@ -94,22 +127,20 @@ enum { MA_ROWS_D = 0, MA_ROWS_PHYS = 1 };
 // inverse transforms run one after another through a single LDS line
 // buffer: a third of the LDS of a lockstep form, so three times the blocks
 // per CU, which is what this strided, bandwidth-bound pass needs.  All
-// global loads are issued before the first transform.
+// global loads are issued before the first transform.  The planes hold every
+// column kx = 0..nx/2 (a.ncols = nx/2 + 1, the context's ncols_full).
 template <typename R, int N, int E, int MODE = MA_ROWS_D>
 __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
     using C = cpx<R>;
     using F = LineFFT<R, N, E>;
     constexpr int T = F::T;
     constexpr int LE = lds_line_elems(N);
-    constexpr int MAXIT = (N / 2 + 1 + T - 1) / T;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
     const int l = threadIdx.x / T, t = threadIdx.x - l * T;
     const int G = a.G;
     const int blk = xcd_block(blockIdx.x, a.nblk);
     const int y0 = blk * 2 * G;
-    const int pitch = a.g.pitch;
-    const int total = a.ncols * G;
     typename F::Twiddles tw;
     tw.load(t, a.twx);
     C* const buf = lds + l * LE;
@@ -117,20 +148,20 @@ __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
     // one run, and consecutive kx are adjacent within a tile
     const int ts = a.yt_log2;
     const size_t t0 = (size_t)ma_tile_off(y0, 0, a.ncols, ts);  // this block's rows, one tile
-    C qa0[MAXIT], qa1[MAXIT], qb0[MAXIT], qb1[MAXIT], qc0[MAXIT], qc1[MAXIT];
-    ma_load_tile<R, N, E, MAXIT>(a.Pxx + t0, ts, G, total, qa0, qa1);
-    ma_load_tile<R, N, E, MAXIT>(a.Pyy + t0, ts, G, total, qb0, qb1);
-    ma_load_tile<R, N, E, MAXIT>(a.Pxy + t0, ts, G, total, qc0, qc1);
-    C w3[3][E];
-    ma_scatter<R, N, E, MAXIT>(lds, G, total, qa0, qa1, 1);
-    F::template run_from_lds<+1>(w3[0], buf, t, tw);
-    __syncthreads();
-    ma_scatter<R, N, E, MAXIT>(lds, G, total, qb0, qb1, 1);
-    F::template run_from_lds<+1>(w3[1], buf, t, tw);
-    __syncthreads();
-    ma_scatter<R, N, E, MAXIT>(lds, G, total, qc0, qc1, 1);
-    F::template run_from_lds<+1>(w3[2], buf, t, tw);
+    C qa0[E / 2], qa1[E / 2], qb0[E / 2], qb1[E / 2], qc0[E / 2], qc1[E / 2];
+    ma_load_tile_full<R, N, E>(a.Pxx + t0, ts, G, qa0, qa1);
+    ma_load_tile_full<R, N, E>(a.Pyy + t0, ts, G, qb0, qb1);
     if constexpr (MODE == MA_ROWS_PHYS) {
+        ma_load_tile_full<R, N, E>(a.Pxy + t0, ts, G, qc0, qc1);
+        C w3[3][E];
+        ma_scatter_full<R, N, E>(lds, G, qa0, qa1);
+        F::template run_from_lds<+1>(w3[0], buf, t, tw);
+        __syncthreads();
+        ma_scatter_full<R, N, E>(lds, G, qb0, qb1);
+        F::template run_from_lds<+1>(w3[1], buf, t, tw);
+        __syncthreads();
+        ma_scatter_full<R, N, E>(lds, G, qc0, qc1);
+        F::template run_from_lds<+1>(w3[2], buf, t, tw);
         const size_t rA = (size_t)(y0 + l) * a.g.nx, rB = (size_t)(y0 + G + l) * a.g.nx;
 #pragma unroll
         for (int f = 0; f < 3; ++f)
@@ -140,31 +171,49 @@ __global__ __launch_bounds__(256) void k_ma_rows(MaRowsArgs<R> a) {
                 a.phys[f][rB + t + T * m] = w3[f][m].y;
             }
         return;
-    }
-    // D = Φxx Φyy - Φxy² for rows A (re) and B (im): compute_d_operator!
-    // (src/poisson.jl:183-223)
-    C v[E];
+    } else {
+        // D = Φxx Φyy - Φxy² for rows A (re) and B (im): compute_d_operator!
+        // (src/poisson.jl:183-223), formed as the transforms finish, so one
+        // product and one transform are live rather than three transforms
+        C v[E], w[E];
+        ma_scatter_full<R, N, E>(lds, G, qa0, qa1);
+        // the third plane's loads wait until the first plane's registers are
+        // free; they are in flight through the first and second transforms
+        ma_load_tile_full<R, N, E>(a.Pxy + t0, ts, G, qc0, qc1);
+        F::template run_from_lds<+1>(v, buf, t, tw);
+        __syncthreads();
+        ma_scatter_full<R, N, E>(lds, G, qb0, qb1);
+        F::template run_from_lds<+1>(w, buf, t, tw);
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-        const C xx = w3[0][m], yy = w3[1][m], xy = w3[2][m];
-        v[m] = C{xx.x * yy.x - xy.x * xy.x, xx.y * yy.y - xy.y * xy.y};
-    }
-    F::template run<-1>(v, buf, t, tw);
-    __syncthreads();
+        for (int m = 0; m < E; ++m) v[m] = C{v[m].x * w[m].x, v[m].y * w[m].y};
+        __syncthreads();
+        ma_scatter_full<R, N, E>(lds, G, qc0, qc1);
+        F::template run_from_lds<+1>(w, buf, t, tw);
 #pragma unroll
-    for (int m = 0; m < E; ++m) buf[lpad(t + T * m)] = v[m];
-    __syncthreads();
-    // D's x-spectrum into the row-tiled output plane, rows A = y0 + l and
-    // B = y0 + G + l
-    C* const dt0 = a.Dx + t0;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int kx = idx >> gsh(G), ll = idx & (G - 1);
-        const C* bl = lds + ll * LE;
-        const C z1 = bl[lpad(kx)];
-        const C z2 = cconj(bl[lpad((N - kx) & (N - 1))]);
-        const unsigned o = ((unsigned)kx << ts) + (unsigned)ll;
-        dt0[o] = C{(z1.x + z2.x) * (R)0.5, (z1.y + z2.y) * (R)0.5};
-        dt0[o + G] = C{(z1.y - z2.y) * (R)0.5, -(z1.x - z2.x) * (R)0.5};
+        for (int m = 0; m < E; ++m) v[m] = C{v[m].x - w[m].x * w[m].x, v[m].y - w[m].y * w[m].y};
+        F::template run<-1>(v, buf, t, tw);
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) buf[lpad(t + T * m)] = v[m];
+        __syncthreads();
+        // D's x-spectrum into the row-tiled output plane, rows A = y0 + l and
+        // B = y0 + G + l: E/2 whole passes over kx < N/2, and the Nyquist
+        // column by the threads of kx = 0
+        C* const dt0 = a.Dx + t0;
+        auto store = [&](int kx, int ll) {
+            const C* bl = lds + ll * LE;
+            const C z1 = bl[lpad(kx)];
+            const C z2 = cconj(bl[lpad((N - kx) & (N - 1))]);
+            const unsigned o = ((unsigned)kx << ts) + (unsigned)ll;
+            dt0[o] = C{(z1.x + z2.x) * (R)0.5, (z1.y + z2.y) * (R)0.5};
+            dt0[o + G] = C{(z1.y - z2.y) * (R)0.5, -(z1.x - z2.x) * (R)0.5};
+        };
+#pragma unroll
+        for (int it = 0; it < E / 2; ++it) {
+            const int idx = threadIdx.x + it * G * T;
+            store(idx >> gsh(G), idx & (G - 1));
+        }
+        if ((int)threadIdx.x < G) store(N / 2, threadIdx.x);
     }
 }
 
