@@ -131,6 +131,10 @@ struct ssg_ctx {
     // that history holds a warm start
     SolveHistory32 ma_hist32;
     bool ma_have_d32 = false;
+    // order of their same-stage warm start: 2 (linear, the default) or 3
+    // (quadratic); SSG_MA_EXTRAP32=3 in the environment selects the quadratic
+    // when a context first solves in FP32.  0: not chosen yet
+    int ma_extrap32 = 0;
     int ma_ept[2] = {0, 0};           // elements per thread of k_ma_rows per path (0 FP64, 1 FP32)
     int ma_cols_ept[2] = {0, 0};      // ... of the Monge-Ampère column passes (0: the tendency's choice)
     Grid g3{};                        // 3D plane geometry: pitch = nz * ny

@@ -1124,6 +1124,13 @@ __device__ __forceinline__ C extrap3(C x, double ca, C y, double cb, C z, double
     const double ry = fma(cc, (double)z.y, fma(cb, (double)y.y, ca * (double)x.y));
     return C{(R)rx, (R)ry};
 }
+// ca x + cb y, as k_extrap forms it with no third entry
+template <typename C>
+__device__ __forceinline__ C extrap2(C x, double ca, C y, double cb) {
+    using R = typename cpx_real_t<C>::type;
+    const double rx = ca * (double)x.x + cb * (double)y.x, ry = ca * (double)x.y + cb * (double)y.y;
+    return C{(R)rx, (R)ry};
+}
 
 // warm-start extrapolation: out = ca a + cb b + cc c (c may be null).  One
 // element per thread (grid of ceil(n / 256) blocks); plane access as in
@@ -1142,8 +1149,7 @@ __global__ __launch_bounds__(256) void k_extrap(C* __restrict__ out, const C* __
         st_plane<R>(out + q, extrap3(x, ca, y, cb, ld_plane<R>(c + q, nt), cc), nt);
         return;
     }
-    const double rx = ca * (double)x.x + cb * (double)y.x, ry = ca * (double)x.y + cb * (double)y.y;
-    st_plane<R>(out + q, C{(R)rx, (R)ry}, nt);
+    st_plane<R>(out + q, extrap2(x, ca, y, cb), nt);
 }
 
 // out = a x + b y (y may be null)

@@ -308,9 +308,10 @@ struct MaTriArgs {
     int nt;                     // nontemporal plane access (ld_plane)
     Grid g;
     // extrapolating form: D̂ is the warm start ring[0] rc[0] + ring[1] rc[1]
-    // + ring[2] rc[2] (extrap3), formed here and also stored to Dg
+    // (+ ring[2] rc[2] when next = 3), formed here and also stored to Dg
     const cpx<R>* ring[3];
     double rc[3];
+    int next;                   // history entries read: 2 (linear) or 3 (quadratic)
     cpx<R>* Dg;
 };
 
@@ -321,9 +322,9 @@ struct MaTriArgs {
 // after exchanging their reduced forms (lane shuffles), and each lane
 // back-substitutes its own half.  Half the registers and half the dependent
 // chain of one Thomas sweep per wavenumber.  NZH >= max(2, ceil(nz/2)).
-// EXT: the extrapolating form, which reads the three history entries in
-// place of D̂, forms the warm start itself and stores it (the residual pass
-// needs it as D̂_prev): one pass over the planes fewer than k_extrap
+// EXT: the extrapolating form, which reads the two or three history entries
+// in place of D̂, forms the warm start itself and stores it (the residual
+// pass needs it as D̂_prev): one pass over the planes fewer than k_extrap
 // followed by this solve, with the same bits.
 template <typename R, typename P, int NZH, bool EXT = false>
 __global__ __launch_bounds__(256) void k_ma_tridiag(MaTriArgs<R, P> a) {
@@ -378,12 +379,18 @@ __global__ __launch_bounds__(256) void k_ma_tridiag(MaTriArgs<R, P> a) {
             }
         };
         if constexpr (EXT) {
-            cpx<R> x1[NZH], x2[NZH];
+            cpx<R> x1[NZH];
             load_all(a.ring[0] + base, dv);
             load_all(a.ring[1] + base, x1);
-            load_all(a.ring[2] + base, x2);
+            if (a.next == 3) {
+                cpx<R> x2[NZH];
+                load_all(a.ring[2] + base, x2);
 #pragma unroll
-            for (int j = 0; j < NZH; ++j) dv[j] = extrap3(dv[j], a.rc[0], x1[j], a.rc[1], x2[j], a.rc[2]);
+                for (int j = 0; j < NZH; ++j) dv[j] = extrap3(dv[j], a.rc[0], x1[j], a.rc[1], x2[j], a.rc[2]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < NZH; ++j) dv[j] = extrap2(dv[j], a.rc[0], x1[j], a.rc[1]);
+            }
             cpx<R>* const g = a.Dg + base;
             if (a.nt) {
 #pragma unroll

@@ -202,15 +202,19 @@ static int ma_stream_planes(const ssg_ctx* c) { return c->ma_stream; }
 // guess is the quadratic extrapolation over the same stage of the last three
 // steps, 3 x[-P] - 3 x[-2P] + x[-3P]: its error is O(dt³), and it carries no
 // error from the RK stage states' own O(dt²) offsets from the trajectory.
-// With less history: the linear 2 x[-1] - x[-2], else x[-1].
+// order = 2 takes the same-stage linear 2 x[-P] - x[-2P] instead.  With less
+// history: the linear 2 x[-1] - x[-2], else x[-1].
 template <typename C>
-int hist_extrapolate(ssg_ctx* c, const SolveHistoryT<C>& h, C* work, int period) {
+int hist_extrapolate(ssg_ctx* c, const SolveHistoryT<C>& h, C* work, int period, int order = 3) {
     const size_t c3 = (size_t)c->ncols_full * c->nz * c->ny;
     const int n = h.n, P = period;
     C* const* r = h.ring;
-    if (P > 0 && n >= 3 * P)
+    if (P > 0 && order == 3 && n >= 3 * P)
         hipLaunchKernelGGL(k_extrap<C>, dim3(grid_all((long)c3)), dim3(256), 0, c->stream, work, r[P - 1], 3.0,
                            r[2 * P - 1], -3.0, r[3 * P - 1], 1.0, (long)c3, ma_stream_planes(c));
+    else if (P > 0 && order == 2 && n >= 2 * P)
+        hipLaunchKernelGGL(k_extrap<C>, dim3(grid_all((long)c3)), dim3(256), 0, c->stream, work, r[P - 1], 2.0,
+                           r[2 * P - 1], -1.0, (const C*)nullptr, 0.0, (long)c3, ma_stream_planes(c));
     else if (n >= 2)
         hipLaunchKernelGGL(k_extrap<C>, dim3(grid_all((long)c3)), dim3(256), 0, c->stream, work, r[0], 2.0, r[1],
                            -1.0, (const C*)nullptr, 0.0, (long)c3, ma_stream_planes(c));
@@ -238,11 +242,12 @@ void ma_history_reset(ssg_ctx* c) {
 constexpr int kMaExtMaxNz = 64;
 
 // Φ̂ = L⁻¹(b̂ + ε D̂) per wavenumber (D̂ term omitted when use_d = 0); with
-// `ext`, D̂ is the quadratic warm start 3 ext[0] - 3 ext[1] + ext[2], formed
-// in the solve and stored to Dh (the extrapolating form of k_ma_tridiag).
+// `ext`, D̂ is the warm start formed in the solve and stored to Dh (the
+// extrapolating form of k_ma_tridiag): the quadratic 3 ext[0] - 3 ext[1] +
+// ext[2] (order 3) or the linear 2 ext[0] - ext[1] (order 2).
 template <typename R, typename P>
 int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d, const cpx<R>* Dh, cpx<P>* Phi,
-               const cpx<R>* const* ext = nullptr) {
+               const cpx<R>* const* ext = nullptr, int order = 3) {
     if (ext && c->nz > kMaExtMaxNz) return set_err(c, SSG_ERR_ARG, "extrapolating solve with nz > %d", kMaExtMaxNz);
     MaTriArgs<R, P> a{};
     a.Dh = Dh;
@@ -259,9 +264,10 @@ int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d, const cpx<R>* Dh, 
     a.g = c->g;
     if (ext) {
         for (int i = 0; i < 3; ++i) a.ring[i] = ext[i];
-        a.rc[0] = 3.0;
-        a.rc[1] = -3.0;
-        a.rc[2] = 1.0;
+        a.next = order == 2 ? 2 : 3;
+        a.rc[0] = order == 2 ? 2.0 : 3.0;
+        a.rc[1] = order == 2 ? -1.0 : -3.0;
+        a.rc[2] = order == 2 ? 0.0 : 1.0;
         a.Dg = const_cast<cpx<R>*>(Dh);
         a.use_d = 1;
     }
@@ -269,7 +275,8 @@ int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d, const cpx<R>* Dh, 
     // two lanes per wavenumber
     const int nblk = (int)((2 * total + 255) / 256);
     const double bytes =
-        (double)total * (c->nz * ((ext ? 4 : 1) * sizeof(cpx<R>) + sizeof(cpx<P>)) + sizeof(cpx<double>));
+        (double)total * (c->nz * ((ext ? (order == 2 ? 3 : 4) : 1) * sizeof(cpx<R>) + sizeof(cpx<P>)) +
+                         sizeof(cpx<double>));
     int slot, rc;
     if ((rc = prof_begin(c, &slot))) return rc;
     auto go = [&](auto kern) { launch_on(c, slot, kern, dim3(nblk), dim3(256), 0, a); };
@@ -292,19 +299,21 @@ int ma_tridiag(ssg_ctx* c, const ssg_ma_params* p, int use_d, const cpx<R>* Dh, 
 }
 
 // The first iterate of a solve warm-started from history h (of `period`
-// solves per step, the working buffer `work`): with three steps of history
-// the quadratic extrapolation is formed inside the solve, otherwise
-// hist_extrapolate runs first (as does no extrapolation without history).
+// solves per step, the working buffer `work`): with `order` steps of
+// history the same-stage extrapolation of that order (3 quadratic, 2 linear)
+// is formed inside the solve, otherwise hist_extrapolate runs first (as does
+// no extrapolation without history).
 template <typename R, typename P>
 int ma_tridiag_warm(ssg_ctx* c, const ssg_ma_params* p, const SolveHistoryT<cpx<R>>& h, int period, cpx<R>* work,
-                    cpx<P>* Phi) {
+                    cpx<P>* Phi, int order = 3) {
     const int P3 = period;
-    if (c->ma_ext_fused && c->nz <= kMaExtMaxNz && P3 > 0 && h.n >= 3 * P3) {
-        const cpx<R>* const ext[3] = {h.ring[P3 - 1], h.ring[2 * P3 - 1], h.ring[3 * P3 - 1]};
-        return ma_tridiag<R, P>(c, p, 1, work, Phi, ext);
+    if (c->ma_ext_fused && c->nz <= kMaExtMaxNz && P3 > 0 && h.n >= order * P3) {
+        const cpx<R>* const ext[3] = {h.ring[P3 - 1], h.ring[2 * P3 - 1],
+                                      h.ring[(order == 2 ? 2 : 3) * P3 - 1]};
+        return ma_tridiag<R, P>(c, p, 1, work, Phi, ext, order);
     }
     int rc;
-    if (h.n > 0 && (rc = hist_extrapolate(c, h, work, period))) return rc;
+    if (h.n > 0 && (rc = hist_extrapolate(c, h, work, period, order))) return rc;
     return ma_tridiag<R, P>(c, p, 1, work, Phi);
 }
 
@@ -889,7 +898,13 @@ static int ma_solve_stage32(ssg_ctx* c, const double* bs, const ssg_ma_params* p
     memset(d, 0, sizeof(*d));
     const double tol32 = std::max(p->tol, (double)SSG_MA_STAGE_TOL32);
     cpx<float>* const pl[3] = {(cpx<float>*)c->ma_p[0], (cpx<float>*)c->ma_p[1], (cpx<float>*)c->ma_p[2]};
-    if ((rc = ma_tridiag_warm<float, float>(c, p, c->ma_hist32, c->ma_period, c->ma32_d, c->ma32_phi))) return rc;
+    if (!c->ma_extrap32) {
+        const char* e = getenv("SSG_MA_EXTRAP32");
+        c->ma_extrap32 = (e && e[0] == '3') ? 3 : 2;
+    }
+    if ((rc = ma_tridiag_warm<float, float>(c, p, c->ma_hist32, c->ma_period, c->ma32_d, c->ma32_phi,
+                                            c->ma_extrap32)))
+        return rc;
     double best = 1e300, ra = 0, rr = 0;
     // the velocities of this iterate are enqueued behind each evaluation's
     // residual reduction, ahead of the host's decision: the iterate is the

@@ -766,6 +766,55 @@ def test_coupled_stage_fp32_rejects(gpu):
     d.close()
 
 
+@pytest.mark.parametrize("order", ["2", "3"])
+def test_stage_fp32_warm_start_order(gpu, order, monkeypatch, capfd):
+    """The FP32 stage solves' same-stage warm start at 1024²×32, ε = 0.1,
+    dt = 2.5e-4, 60 RK3 steps in two calls.  Linear (order 2, the default):
+    once two steps of history are held every FP32 stage solve converges on
+    its first evaluation.  The quadratic (order 3, SSG_MA_EXTRAP32=3) is
+    kept as an option; its accepted one-evaluation iterates feed an error
+    recurrence that grows from step to step until a solve needs a second
+    evaluation, so it takes at least as many.  Both stay within 1e-5 of the
+    FP64 coupled step (b, φₛ, surface kinetic energy)."""
+    n, nz, eps, dt = 1024, 32, 0.1, 2.5e-4
+    b0 = smooth_b(n, 0, 6.0)
+
+    def run(prec, s32):
+        d = gpu.make_domain(n, n, nz, precision=prec)
+        f = gpu.allocate_fields(d)
+        f.bs = b0
+        gpu.set_coupling(d, eps=eps, tol=1e-10, maxiter=60, variant=gpu.MA_BODY, warm=2, stage_fp32=s32)
+        st = gpu.TimeState(0.0, f)
+        p = gpu.TimeParams(dt, scheme=gpu.RK3, filter_freq=10, filter_strength=36.0, filter_order=8,
+                           filter_cutoff=0.65)
+        for k in (30, 30):
+            gpu.timestep(f, d, p, st, k)
+        out = (f.bs, f.phis, gpu.diagnostics(d)["ke_surface"])
+        d.close()
+        return out
+
+    b64, p64, k64 = run(gpu.SSG_FP64, False)
+    monkeypatch.setenv("SSG_MA_TRACE", "1")
+    if order == "3":
+        monkeypatch.setenv("SSG_MA_EXTRAP32", "3")
+    capfd.readouterr()
+    b32, p32, k32 = run(gpu.SSG_FP32_MIXED, True)
+    trace = [ln.split() for ln in capfd.readouterr().err.splitlines() if ln.startswith("ssg_ma_trace")]
+    fp32 = [t for t in trace if t[1] == "fp32"]
+    assert all(t[6] == "conv=1" for t in fp32)
+    evals = [int(t[2].split("=")[1]) for t in fp32]
+    print(f"order {order}: {len(evals)} FP32 stage solves, {sum(evals)} evaluations, "
+          f"{sum(e > 1 for e in evals)} with more than one", flush=True)
+    if order == "2":
+        # the first call's first two steps build the history (5 FP32 solves)
+        assert all(e == 1 for e in evals[6:]), evals
+    eb = np.linalg.norm(b32 - b64) / np.linalg.norm(b64)
+    ep = np.linalg.norm(p32 - p64) / np.linalg.norm(p64)
+    ek = abs(k32 - k64) / abs(k64)
+    print(f"order {order}: b {eb:.2e} phis {ep:.2e} ke {ek:.2e}", flush=True)
+    assert 0 < eb < 1e-5 and ep < 1e-5 and ek < 1e-5
+
+
 def test_coupled_stage_fp32_full_size(gpu):
     """The precision configuration's grid, 1024²×32, ε = 0.1, dt = 2.5e-4:
     60 coupled RK3 steps (two calls) with FP32 stage solves against the FP64
