@@ -520,7 +520,7 @@ def main():
     ap.add_argument("--c1-steps", type=int, default=200,
                     help="also run the reference's own CPU case (128²×16, set_b! then RK3 dt = 1e-3) for this "
                          "many steps on the GPU and, unless --no-cpu, on the CPU oracle; 0 = skip")
-    ap.add_argument("--ensemble-steps", type=int, default=100,
+    ap.add_argument("--ensemble-steps", type=int, default=500,
                     help="also time the 64-member batched configuration for this many steps; 0 = skip")
     ap.add_argument("--fft-compare", action="store_true",
                     help="only time the spectral round trip (rfft2 -> diagonal -> irfft2) of this library "
